@@ -1,0 +1,166 @@
+"""ctypes wrapper of the CPU oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by the smoke() entry point and by
+bench.py's cpu_baseline leg, never by the product package.  The C source
+(ldpc_oracle.c) cites the reference file:line each function restates.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from typing import Tuple
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+REF_STEPPER = os.path.join(HERE, "_ref", "stepper")
+
+_lib = None
+
+u16p = np.ctypeslib.ndpointer(np.uint16, flags="C_CONTIGUOUS")
+u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
+i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+u8p = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
+u64p = np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS")
+
+
+def build(force: bool = False) -> str:
+    if force or not os.path.exists(LIB_PATH):
+        subprocess.run(["make", "-s", "-C", HERE] + (["-B"] if force else []), check=True)
+    return LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        build()
+        L = C.CDLL(LIB_PATH)
+        L.orc_amin_t.restype = C.c_float
+        L.orc_amin_t.argtypes = [C.c_float, C.c_float]
+        L.orc_decode_t.restype = C.c_int
+        L.orc_decode_t.argtypes = [C.c_int, C.c_int, i32p, i32p, i32p, C.c_int, u16p, C.c_int, C.c_float,
+                                   u16p, u32p, u8p, C.c_int]
+        L.orc_trace_t.restype = C.c_int
+        L.orc_trace_t.argtypes = [C.c_int, C.c_int, i32p, i32p, i32p, u16p, C.c_int, C.c_float, u32p, u16p]
+        L.orc_philox4x32_10.restype = None
+        L.orc_philox4x32_10.argtypes = [u32p, u32p, u32p]
+        L.orc_log_unit.restype = C.c_float
+        L.orc_log_unit.argtypes = [C.c_float]
+        L.orc_sqrt_newton.restype = C.c_float
+        L.orc_sqrt_newton.argtypes = [C.c_float]
+        L.orc_sincos_turn.restype = None
+        L.orc_sincos_turn.argtypes = [C.c_uint32, C.POINTER(C.c_float), C.POINTER(C.c_float)]
+        L.orc_box_muller.restype = None
+        L.orc_box_muller.argtypes = [C.c_uint32, C.c_uint32, C.POINTER(C.c_float), C.POINTER(C.c_float)]
+        L.orc_gen_channel.restype = None
+        L.orc_gen_channel.argtypes = [C.c_uint64, C.c_uint64, C.c_int, C.c_float, C.c_float, C.c_int, C.c_int,
+                                      u16p]
+        L.orc_run_point_t.restype = C.c_int
+        L.orc_run_point_t.argtypes = [C.c_int, C.c_int, i32p, i32p, i32p, C.c_int, C.c_int, C.c_uint64,
+                                      C.c_uint64, C.c_uint64, C.c_float, C.c_float, C.c_int, C.c_float, C.c_int,
+                                      u64p, u64p]
+        L.orc_num_threads.restype = C.c_int
+        L.orc_num_threads.argtypes = []
+        _lib = L
+    return _lib
+
+
+def code_arrays(code) -> Tuple[np.ndarray, np.ndarray, np.ndarray]:
+    """(row_ptr, col, sched) int32 arrays of a codes.Code."""
+    rp = np.zeros(code.m + 1, dtype=np.int32)
+    rp[1:] = np.cumsum([len(r) for r in code.rows])
+    col = np.concatenate([np.asarray(r, dtype=np.int32) for r in code.rows]).astype(np.int32)
+    sched = code.schedule_rows().astype(np.int32)
+    return rp, col, sched
+
+
+def amin_t(a: float, b: float) -> float:
+    return lib().orc_amin_t(a, b)
+
+
+def decode_t(code, llr: np.ndarray, max_iter: int = 200, beta: float = 0.9375, threads: int = 0):
+    llr = np.ascontiguousarray(llr, dtype=np.uint16).reshape(-1, code.n)
+    n_cw = llr.shape[0]
+    rp, col, sched = code_arrays(code)
+    out = np.empty_like(llr)
+    iters = np.empty(n_cw, dtype=np.uint32)
+    ok = np.empty(n_cw, dtype=np.uint8)
+    rc = lib().orc_decode_t(code.n, code.m, rp, col, sched, n_cw, llr, max_iter, beta, out, iters, ok, threads)
+    if rc:
+        raise ValueError("oracle Profile T supports check degrees 3 and 6 only")
+    return out, iters, ok
+
+
+def trace_t(code, llr_one: np.ndarray, max_iter: int = 200, beta: float = 0.9375):
+    llr_one = np.ascontiguousarray(llr_one, dtype=np.uint16).reshape(code.n)
+    rp, col, sched = code_arrays(code)
+    unsat = np.zeros(max_iter, dtype=np.uint32)
+    out = np.empty_like(llr_one)
+    it = lib().orc_trace_t(code.n, code.m, rp, col, sched, llr_one, max_iter, beta, unsat, out)
+    return unsat[:it], out
+
+
+def philox4x32_10(ctr, key) -> np.ndarray:
+    out = np.zeros(4, dtype=np.uint32)
+    lib().orc_philox4x32_10(np.asarray(ctr, dtype=np.uint32), np.asarray(key, dtype=np.uint32), out)
+    return out
+
+
+def log_unit(u: float) -> float:
+    return lib().orc_log_unit(u)
+
+
+def sqrt_newton(t: float) -> float:
+    return lib().orc_sqrt_newton(t)
+
+
+def sincos_turn(x: int):
+    s, c = C.c_float(), C.c_float()
+    lib().orc_sincos_turn(x, C.byref(s), C.byref(c))
+    return s.value, c.value
+
+
+def box_muller(x0: int, x1: int):
+    a, b = C.c_float(), C.c_float()
+    lib().orc_box_muller(x0, x1, C.byref(a), C.byref(b))
+    return a.value, b.value
+
+
+def gen_channel(code, seed: int, first_cw: int, n_cw: int, mu: float, sigma: float) -> np.ndarray:
+    out = np.empty((n_cw, code.n), dtype=np.uint16)
+    lib().orc_gen_channel(seed, first_cw, n_cw, mu, sigma, code.n_tx, code.n, out)
+    return out
+
+
+def run_point_t(code, seed: int, first_cw: int, n_cw: int, mu: float, sigma: float, max_iter: int = 200,
+                beta: float = 0.9375, threads: int = 0):
+    """Returns dict(blocks, frame_errors, bit_errors, iter_sum, hist)."""
+    rp, col, sched = code_arrays(code)
+    cnt = np.zeros(4, dtype=np.uint64)
+    hist = np.zeros(max_iter + 1, dtype=np.uint64)
+    rc = lib().orc_run_point_t(code.n, code.m, rp, col, sched, code.n_tx, code.k, seed, first_cw, n_cw, mu, sigma,
+                               max_iter, beta, threads, cnt, hist)
+    if rc:
+        raise ValueError("oracle Profile T supports check degrees 3 and 6 only")
+    return dict(blocks=int(cnt[0]), frame_errors=int(cnt[1]), bit_errors=int(cnt[2]), iter_sum=int(cnt[3]),
+                hist=hist)
+
+
+def num_threads() -> int:
+    return lib().orc_num_threads()
+
+
+# --- Packed4CW dump format (stepper:879-880, 962-965) ------------------------
+
+def to_packed4(llr4: np.ndarray) -> bytes:
+    """[4, n] bf16 codewords -> variable-major Packed4CW bytes (n * 8 B)."""
+    llr4 = np.asarray(llr4, dtype=np.uint16)
+    assert llr4.shape[0] == 4
+    return np.ascontiguousarray(llr4.T).astype("<u2").tobytes()
+
+
+def from_packed4(buf: bytes, n: int) -> np.ndarray:
+    a = np.frombuffer(buf, dtype="<u2", count=4 * n).reshape(n, 4)
+    return np.ascontiguousarray(a.T)

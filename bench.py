@@ -1,0 +1,196 @@
+#!/usr/bin/env python3
+"""Throughput of the Monte Carlo LDPC hot path on one MI355X.
+
+Workload: CCSDS AR4JA rate-1/2 K=1024 (N=2560, 512 punctured, 2048 BPSK
+symbols per codeword), Eb/N0 = 2.0 dB, Profile T (the reference golden's tree
+Approx-Min*, beta = 15/16), 200 max iterations with syndrome early exit.  One
+step is one fused launch of 2^20 codewords: Philox/Box-Muller channel
+generation -> layered decoding -> FER/BER/iteration tallies, all on device
+(the launch shape of the error-floor configuration).  Synthetic data: the
+all-zero codeword plus AWGN from the counter-based stream.
+
+Prints ONE JSON line.  value = coded channel symbols per second over the timed
+wall clock (Gsps).  roofline = VALU issue (the decoder keeps each codeword on
+chip; HBM carries only counters), from HIP-event kernel time measured on the
+launch stream.  cpu_baseline = the reference golden stepper (built from the
+reference's source into oracle/_ref) or, if absent, the CPU oracle, timed on
+this host's cores on a bounded sample of the same workload.
+"""
+import argparse
+import json
+import math
+import os
+import re
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path[:0] = [os.path.join(ROOT, "tt-ldpc-qb2-sim_amd"), os.path.join(ROOT, "oracle")]
+
+EBN0_DB = 2.0
+MAX_ITER = 200
+BETA = 0.9375
+CW_PER_STEP = 1 << 20
+N_TX = 2048
+# Algorithmic work per codeword-iteration (7680 edge updates), Profile T:
+# VALU lane-ops (deg-6 check 239, deg-3 check 86; 1024*239 + 512*86) and LDS
+# bytes (10 B per edge: L read/write, R read/write, syndrome read).
+OPS_PER_CWI = 1024 * 239 + 512 * 86
+LDS_BYTES_PER_CWI = 7680 * 10
+# MI355X: 256 CUs x 4 SIMD x 16 lanes x 2.4 GHz (non-packed VALU);
+# LDS 128 B/clk/CU; HBM3E 8 TB/s.
+PEAK_VALU_TOPS = 256 * 4 * 16 * 2.4e9 / 1e12
+PEAK_LDS_TBS = 256 * 128 * 2.4e9 / 1e12
+
+
+def poisson_ci95(k: int):
+    """Exact (Garwood) 95% interval of a Poisson count."""
+    from scipy.stats import chi2
+    lo = 0.0 if k == 0 else chi2.ppf(0.025, 2 * k) / 2
+    hi = chi2.ppf(0.975, 2 * k + 2) / 2
+    return lo, hi
+
+
+def cpu_reference_baseline(seconds: float):
+    """Reference golden stepper, --sfpi --monte-carlo, all host threads."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "stepper")
+    if not os.path.exists(exe):
+        return None
+    threads = os.cpu_count() or 1
+    # sized from a short probe so the sample takes about `seconds`
+    probe = 400 * threads
+    t0 = time.time()
+    p = subprocess.run([exe, "--sfpi", "--monte-carlo", str(EBN0_DB), str(probe), "1000000000", str(MAX_ITER),
+                        str(threads)], capture_output=True, text=True, timeout=600)
+    dt = time.time() - t0
+    blocks = max(int(probe * seconds / max(dt, 1e-3)) // 4 * 4, probe)
+    p = subprocess.run([exe, "--sfpi", "--monte-carlo", str(EBN0_DB), str(blocks), "1000000000", str(MAX_ITER),
+                        str(threads)], capture_output=True, text=True, timeout=900)
+    m = re.search(r"Host Throughput:\s*([0-9.]+) Msps", p.stdout)
+    fe = re.search(r"Frame Errors:\s*(\d+)", p.stdout)
+    tb = re.search(r"Total Blocks:\s*(\d+)", p.stdout)
+    it = re.search(r"Mean Iterations:\s*([0-9.]+)", p.stdout)
+    if not m:
+        return None
+    return {"value": float(m.group(1)) / 1e3, "unit": "Gsps", "cores": threads, "kind": "reference",
+            "sample": f"reference golden stepper (tests/test_circulant_dma_stepper.cpp, built from source) "
+                      f"--sfpi --monte-carlo {EBN0_DB} dB, {tb.group(1)} codewords, 200 max iter, "
+                      f"{threads} OpenMP threads; mean iterations {it.group(1) if it else '?'}, "
+                      f"FE {fe.group(1) if fe else '?'}"}
+
+
+def cpu_oracle_baseline(seconds: float):
+    import numpy as np  # noqa: F401
+
+    import oracle as O
+    from ldpc_mi355x import codes
+    code = codes.ar4ja(1024, "1/2")
+    g = 10 ** (EBN0_DB / 10)
+    mu, sg = 4 * 0.5 * g, math.sqrt(8 * 0.5 * g)
+    threads = O.num_threads()
+    blocks, t0, first = 0, time.time(), 0
+    while time.time() - t0 < seconds:
+        n = 512 * threads
+        O.run_point_t(code, 0xC0DE, first, n, mu, sg, MAX_ITER, BETA, threads)
+        blocks += n
+        first += n
+    dt = time.time() - t0
+    return {"value": blocks * N_TX / dt / 1e9, "unit": "Gsps", "cores": threads, "kind": "port",
+            "sample": f"CPU oracle (oracle/ldpc_oracle.c, Profile T), {blocks} codewords at {EBN0_DB} dB, "
+                      f"200 max iter, {threads} OpenMP threads"}
+
+
+def read_traffic():
+    """HBM bytes per launch from the committed rocprofv3 --pmc summary, if any."""
+    path = os.path.join(ROOT, "profiles", "k_monte_carlo_r12_k1024_2p0db_pmc.txt")
+    if not os.path.exists(path):
+        return None
+    m = re.search(r"hbm_bytes_per_launch\s*=\s*([0-9.eE+]+)", open(path).read())
+    return float(m.group(1)) if m else None
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build measures one GPU only (the decode path shards by codeword id across GPUs "
+              "but no multi-GPU figure is produced); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    from ldpc_mi355x import decoder
+    eng = decoder.Engine(1024, "1/2", max_iter=MAX_ITER, beta=BETA, device=0)
+    seed = 0x5EED2026
+    first = 0
+    for _ in range(args.warmup):
+        r = eng.run_point(EBN0_DB, CW_PER_STEP, seed=seed, first_cw=first, chunk_cw=CW_PER_STEP)
+        first = r.next_cw
+    eng.sync()
+    acc = None
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        acc = eng.run_point(EBN0_DB, CW_PER_STEP, seed=seed, first_cw=first, chunk_cw=CW_PER_STEP, result=acc)
+        first = acc.next_cw
+    eng.sync()
+    wall = time.perf_counter() - t0
+
+    value = acc.blocks * N_TX / wall / 1e9
+    kernel_s = acc.kernel_s
+    per_launch_s = kernel_s / acc.launches
+    cwi_per_launch = acc.iter_sum / acc.launches
+    ops_tops = OPS_PER_CWI * cwi_per_launch / per_launch_s / 1e12
+    lds_tbs = LDS_BYTES_PER_CWI * cwi_per_launch / per_launch_s / 1e12
+    fe_lo, fe_hi = poisson_ci95(acc.frame_errors)
+
+    cpu = None
+    if not args.no_cpu:
+        try:
+            cpu = cpu_reference_baseline(args.cpu_seconds) or cpu_oracle_baseline(args.cpu_seconds)
+        except Exception as e:  # a baseline failure must not hide the GPU number
+            cpu = {"value": None, "unit": "Gsps", "cores": os.cpu_count(), "kind": "reference",
+                   "sample": f"failed: {e}"}
+
+    out = {
+        "metric": "Gsps (channel sym/s) 1 GPU, AR4JA r1/2 K=1024 @2.0dB; FER-matched speedup",
+        "value": value,
+        "unit": "Gsps",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16",
+        "data": "synthetic: all-zero codeword, BPSK, AWGN from Philox4x32-10 + Box-Muller generated on device",
+        "config": {
+            "workload": "AR4JA r1/2 K=1024 (N=2560, 2048 tx symbols), Eb/N0 2.0 dB, 200 max iterations, "
+                        "Profile T (golden tree Approx-Min*, beta 15/16), 2^20 codewords per step",
+            "code": "AR4JA r1/2 K=1024", "ebn0_db": EBN0_DB, "max_iter": MAX_ITER, "beta": BETA,
+            "codewords_per_step": CW_PER_STEP, "symbols_per_codeword": N_TX,
+        },
+        "roofline": {
+            "bound": "valu", "achieved": ops_tops, "peak": PEAK_VALU_TOPS, "unit": "Tlane-op/s",
+            "frac": ops_tops / PEAK_VALU_TOPS, "traffic": read_traffic(),
+            "kernel": "k_monte_carlo", "kernel_ms_per_launch": per_launch_s * 1e3,
+            "algorithmic_ops_per_launch": OPS_PER_CWI * cwi_per_launch,
+            "lds": {"achieved": lds_tbs, "peak": PEAK_LDS_TBS, "unit": "TB/s", "frac": lds_tbs / PEAK_LDS_TBS},
+        },
+        "cpu_baseline": cpu,
+        "fer": acc.fer, "fer_ci95": [fe_lo / acc.blocks, fe_hi / acc.blocks], "frame_errors": acc.frame_errors,
+        "ber": acc.ber, "mean_iterations": acc.mean_iterations, "blocks": acc.blocks,
+        "kernel_gsps": acc.blocks * N_TX / kernel_s / 1e9,
+        "speedup_vs_cpu": (value / cpu["value"]) if cpu and cpu.get("value") else None,
+        "reference_published_gsps": 5.2873,
+    }
+    print(json.dumps(out), flush=True)
+    eng.close()
+
+
+if __name__ == "__main__":
+    main()

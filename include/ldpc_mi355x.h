@@ -1,0 +1,169 @@
+/*
+ * ldpc_mi355x.h -- C ABI of libldpc, the MI355X (gfx950) Monte Carlo LDPC
+ * engine for the CCSDS AR4JA codes.
+ *
+ * The reference drives its decoder as a separate process: run_simulation.py
+ * builds an argv for the engine binary and reads JSON lines from its stdout
+ * (run_simulation.py:165-225; engines/scalar_5w/ldpc_sim_scalar.cpp:271-641).
+ * This library is the in-process replacement of that engine: a Python or C
+ * host calls these functions through ctypes/dlopen, with plain pointers and
+ * fixed-width POD structs.  Each entry point names the reference interface it
+ * replaces.
+ *
+ * Conventions:
+ *   - Every function returns LDPC_OK (0) or a negative ldpc_status; none
+ *     throws across the ABI.  ldpc_last_error() gives the message.
+ *   - Host buffers are caller-owned; device memory, streams and events are
+ *     owned by the context.  Calls are synchronous unless named *_async.
+ *   - A context is not thread-safe: use one context per host thread.
+ *   - LLRs are bfloat16 bit patterns (uint16_t), codeword-major, N per
+ *     codeword; punctured variables are the last ones of each codeword.
+ */
+#ifndef LDPC_MI355X_H
+#define LDPC_MI355X_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LDPC_ABI_VERSION 1
+
+typedef enum ldpc_status {
+    LDPC_OK = 0,
+    LDPC_ERR_ARG = -1,         /* invalid argument */
+    LDPC_ERR_UNSUPPORTED = -2, /* code / profile combination not built */
+    LDPC_ERR_HIP = -3,         /* HIP runtime error */
+    LDPC_ERR_NOMEM = -4,       /* allocation failed */
+    LDPC_ERR_NODEV = -5        /* no usable GPU */
+} ldpc_status;
+
+/* Decoder arithmetic profiles. */
+enum {
+    /* Tree Approximate-Min* with bf16 rounding of q, r and L, truncated
+     * constants and damping beta: the reference's golden stepper
+     * (tests/test_circulant_dma_stepper.cpp:107-327, :743-815). */
+    LDPC_PROFILE_T = 0
+};
+
+/* Code families. */
+enum {
+    LDPC_CODE_AR4JA = 1 /* CCSDS AR4JA (matrices/makeH_AR4JA.m) */
+};
+
+typedef struct ldpc_code_desc {
+    uint32_t family;   /* LDPC_CODE_AR4JA */
+    uint32_t k_info;   /* 1024, 4096 or 16384 */
+    uint32_t rate_num; /* 1, 2 or 4 */
+    uint32_t rate_den; /* 2, 3 or 5 */
+} ldpc_code_desc;
+
+typedef struct ldpc_decoder_cfg {
+    uint32_t profile;  /* LDPC_PROFILE_T */
+    uint32_t max_iter; /* >= 1 (reference default 200) */
+    float beta;        /* damping of check messages; 0.9375 matches the
+                          silicon and the stepper's --load-init default
+                          (compute_trisc.cpp:395,566; stepper:1017) */
+    uint32_t reserved[5];
+} ldpc_decoder_cfg;
+
+typedef struct ldpc_code_info {
+    uint32_t n;       /* variables (transmitted + punctured) */
+    uint32_t m;       /* checks */
+    uint32_t k_info;  /* information bits (the first k_info variables) */
+    uint32_t n_tx;    /* transmitted symbols per codeword */
+    uint32_t z;       /* layer (circulant) size */
+    uint32_t n_layers;
+    uint32_t n_edges;
+    uint32_t max_iter;
+} ldpc_code_info;
+
+/* One Monte Carlo operating point: all-zero codeword, BPSK, AWGN. */
+typedef struct ldpc_point_cfg {
+    double ebn0_db;
+    uint64_t seed;        /* Philox key */
+    uint64_t first_cw;    /* codeword id of the first codeword simulated */
+    uint64_t max_blocks;  /* stop after this many codewords */
+    uint64_t target_fe;   /* stop once this many frame errors (0: none) */
+    uint64_t chunk_cw;    /* codewords per launch (0: 1 << 20; at most 1 << 31) */
+} ldpc_point_cfg;
+
+typedef struct ldpc_point_result {
+    uint64_t blocks;       /* codewords simulated */
+    uint64_t frame_errors; /* codewords whose syndrome never cleared */
+    uint64_t bit_errors;   /* info bits <= 0 in those codewords */
+    uint64_t iter_sum;     /* sum of iterations (max_iter for failures) */
+    uint64_t next_cw;      /* first codeword id not yet simulated */
+    uint64_t elapsed_ns;   /* host wall time of the call */
+    uint64_t kernel_ns;    /* summed decode-kernel time (HIP events) */
+    uint64_t launches;     /* decode-kernel launches */
+} ldpc_point_result;
+
+typedef struct ldpc_ctx ldpc_ctx;
+
+/* Create a context on a device for one code and decoder configuration.
+ * Replaces the engine binary's start-up: matrix parse and device open
+ * (ldpc_sim_scalar.cpp:271-330; ldpc_sim_circulant_dma.cpp:98-126). */
+int ldpc_create(const ldpc_code_desc *code, const ldpc_decoder_cfg *cfg, int device, ldpc_ctx **out);
+
+void ldpc_destroy(ldpc_ctx *ctx);
+
+/* Last error message of ctx (or of the last failed ldpc_create if ctx is
+ * NULL).  The string lives until the next call on the same context. */
+const char *ldpc_last_error(const ldpc_ctx *ctx);
+
+int ldpc_get_code_info(const ldpc_ctx *ctx, ldpc_code_info *info);
+
+/* Layer schedule compiled into the kernels for a code (needs no GPU): per
+ * layer [first_row, deg, (base, shift) * deg], layers in decoding order;
+ * variable of check j of a layer = base + (shift + j) mod z.  Returns the
+ * number of int32 written or a negative status. */
+int ldpc_get_schedule(const ldpc_code_desc *code, int32_t *out, int32_t cap);
+
+/* Monte Carlo point: generate, decode and tally codewords first_cw.. until
+ * max_blocks or target_fe is reached, in launches of chunk_cw codewords.
+ * hist (may be NULL) has max_iter + 1 entries: hist[i] = codewords that
+ * converged at iteration i, hist[0] = codewords that did not converge.
+ * Results ADD to *res and hist, so a sweep can be resumed chunk by chunk.
+ * Replaces the engine's batch loop and stopping rule
+ * (ldpc_sim_scalar.cpp:421-641; ldpc_sim_circulant_dma.cpp:295-411) and the
+ * golden Monte Carlo (stepper:708-856). */
+int ldpc_run_point(ldpc_ctx *ctx, const ldpc_point_cfg *pt, ldpc_point_result *res, uint64_t *hist);
+
+/* Decode caller-supplied channel LLRs (host buffers, n_cw * N bf16 each).
+ * out_iters[c] = iteration of convergence, or max_iter if it never converged;
+ * out_ok[c] = 1 if the syndrome cleared.  Replaces the stepper's dump replay
+ * (stepper:861-990). */
+int ldpc_decode_llrs(ldpc_ctx *ctx, const uint16_t *llr, uint32_t n_cw, uint16_t *out_llr, uint32_t *out_iters,
+                     uint8_t *out_ok);
+
+/* Same on device buffers, enqueued on the context's stream; returns after
+ * the launch (use ldpc_sync).  Buffers must stay valid until then. */
+int ldpc_decode_llrs_device_async(ldpc_ctx *ctx, const void *d_llr, uint32_t n_cw, void *d_out_llr,
+                                  void *d_out_iters, void *d_out_ok);
+
+/* Channel LLRs of codewords first_cw .. first_cw + n_cw - 1 at ebn0_db
+ * (all-zero codeword, BPSK, AWGN from Philox4x32-10 keyed by seed), written
+ * to a host buffer of n_cw * N bf16.  Replaces the device channel generator
+ * (compute_trisc.cpp:179-294; ldpc_decoder_core.h:128-152). */
+int ldpc_gen_channel(ldpc_ctx *ctx, uint64_t seed, uint64_t first_cw, uint32_t n_cw, double ebn0_db,
+                     uint16_t *out_llr);
+
+/* BPSK LLR mean and standard deviation at ebn0_db for this code's rate
+ * R = k_info / n_tx:  mu = 4 R g, sigma = sqrt(8 R g), g = 10^(ebn0/10)
+ * (ldpc_sim_scalar.cpp:352-355). */
+int ldpc_channel_params(const ldpc_ctx *ctx, double ebn0_db, float *mu, float *sigma);
+
+/* Wait for all work on the context's stream. */
+int ldpc_sync(ldpc_ctx *ctx);
+
+/* The context's HIP stream (hipStream_t), for callers that time or order
+ * work on it. */
+void *ldpc_stream(ldpc_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LDPC_MI355X_H */

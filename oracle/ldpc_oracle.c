@@ -330,6 +330,25 @@ static void gen_one(const uint32_t key[2], uint64_t w, float mu, float sigma, in
     for (int v = n_tx; v < n; ++v) L[v] = 0;
 }
 
+/* The standard normals behind gen_one (z before mu + sigma z), n_tx per codeword. */
+EXPORT void orc_normals(uint64_t seed, uint64_t first_cw, int n_cw, int n_tx, float *out) {
+    const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
+    for (int c = 0; c < n_cw; ++c) {
+        uint64_t w = first_cw + (uint64_t)c;
+        for (int blk = 0; blk < (n_tx + 3) / 4; ++blk) {
+            uint32_t ctr[4] = {(uint32_t)w, (uint32_t)(w >> 32), (uint32_t)blk, 0u}, x[4];
+            orc_philox4x32_10(ctr, key, x);
+            float z[4];
+            orc_box_muller(x[0], x[1], &z[0], &z[1]);
+            orc_box_muller(x[2], x[3], &z[2], &z[3]);
+            for (int j = 0; j < 4 && 4 * blk + j < n_tx; ++j) out[(size_t)c * n_tx + 4 * blk + j] = z[j];
+        }
+    }
+}
+
 EXPORT void orc_gen_channel(uint64_t seed, uint64_t first_cw, int n_cw, float mu, float sigma,
                             int n_tx, int n, uint16_t *out) {
     const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};

@@ -57,6 +57,9 @@ def lib():
         L.orc_gen_channel.restype = None
         L.orc_gen_channel.argtypes = [C.c_uint64, C.c_uint64, C.c_int, C.c_float, C.c_float, C.c_int, C.c_int,
                                       u16p]
+        L.orc_normals.restype = None
+        L.orc_normals.argtypes = [C.c_uint64, C.c_uint64, C.c_int, C.c_int,
+                                  np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")]
         L.orc_run_point_t.restype = C.c_int
         L.orc_run_point_t.argtypes = [C.c_int, C.c_int, i32p, i32p, i32p, C.c_int, C.c_int, C.c_uint64,
                                       C.c_uint64, C.c_uint64, C.c_float, C.c_float, C.c_int, C.c_float, C.c_int,
@@ -131,6 +134,12 @@ def box_muller(x0: int, x1: int):
 def gen_channel(code, seed: int, first_cw: int, n_cw: int, mu: float, sigma: float) -> np.ndarray:
     out = np.empty((n_cw, code.n), dtype=np.uint16)
     lib().orc_gen_channel(seed, first_cw, n_cw, mu, sigma, code.n_tx, code.n, out)
+    return out
+
+
+def normals(seed: int, first_cw: int, n_cw: int, n_tx: int) -> np.ndarray:
+    out = np.empty((n_cw, n_tx), dtype=np.float32)
+    lib().orc_normals(seed, first_cw, n_cw, n_tx, out)
     return out
 
 

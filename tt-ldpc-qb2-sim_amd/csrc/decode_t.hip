@@ -1,0 +1,410 @@
+// decode_t.hip -- Profile T layered Approximate-Min* decoder for gfx950.
+//
+// Bit-exact restatement of the reference's golden layered decoder
+// (tests/test_circulant_dma_stepper.cpp:107-327 check math, :743-815 schedule,
+// syndrome, latch and tallies) for AR4JA rate-1/2 K=1024, mapped to CDNA4:
+//
+//  * one codeword per wave64; the wave's 2560 variable LLRs (bf16) live in
+//    LDS (5 KiB), the 7680 check-to-variable messages R live in VGPRs as
+//    packed bf16 pairs (60 dwords per lane);
+//  * a layer is one quarter of a block row (128 checks, no shared variable);
+//    lane l updates checks l and l+64 of the layer, so every edge slot of a
+//    layer is one cyclic-shifted 128-variable read/modify/write in LDS;
+//  * layers run in Gauss-Seidel order with the 12-layer schedule unrolled at
+//    compile time; LDS operations of one wave execute in issue order, so no
+//    barrier separates layers (only a compiler fence);
+//  * the syndrome is computed after every iteration from 40 wave ballots of
+//    the hard decisions (one 128-bit mask per variable quarter) rotated and
+//    XOR-ed on the scalar unit;
+//  * converged codewords leave the loop (convergence latch); a wave then
+//    takes the next codeword id from a device work queue.
+//
+// Arithmetic per edge (fp32, every op separately rounded, RNE to bf16 with
+// the reference's integer bias formula):
+//   q   = RNE(L - R)                          (stepper:112-116)
+//   r_d = tree min* over the other edges, * beta (stepper:151-225; :300-304)
+//   out = RNE(copysign(r_d, s ^ sign(q_d)))   (stepper:178-186)
+//   L'  = RNE(q + out),  R' = out              (stepper:179-180)
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <utility>
+
+#include "ar4ja_schedule.h"
+#include "device_math.h"
+#include "kernels.h"
+
+#pragma clang fp contract(off)
+
+namespace ldpc {
+namespace {
+
+constexpr ar4ja::Schedule S12 = ar4ja::make_schedule(0, 0);  // r1/2, K = 1024
+constexpr int Z = 128;
+constexpr int NV = 2560;
+constexpr int NTX = 2048;
+constexpr int KINFO = 1024;
+constexpr int NLAYERS = 12;
+constexpr int WAVES_PER_WG = 4;
+static_assert(S12.Z == Z && S12.N == NV && S12.N - S12.n_punct == NTX && S12.k_info == KINFO, "schedule");
+
+constexpr int r_offset(int li) {
+    int o = 0;
+    for (int i = 0; i < li; ++i) o += S12.layer[i].deg;
+    return o;
+}
+constexpr int NR = r_offset(NLAYERS);  // 60 packed dwords per lane
+static_assert(NR == 60, "R register budget");
+
+__device__ __forceinline__ float ld_bf(const uint16_t *p) { return u2f((uint32_t)(*p) << 16); }
+
+// Compiler-only ordering point between layers.  LDS requests of one
+// wavefront are performed in issue order by the hardware, so a store by one
+// lane is seen by a later load of another lane of the same wave; the fence
+// keeps the compiler from hoisting the next layer's loads above the stores.
+__device__ __forceinline__ void wave_lds_fence() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront"); }
+
+template <int D>
+__device__ __forceinline__ void check_tree(const float (&q)[D], float beta, float (&r)[D]) {
+    if constexpr (D == 6) {
+        float m0 = fabsf(q[0]), m1 = fabsf(q[1]), m2 = fabsf(q[2]);
+        float m3 = fabsf(q[3]), m4 = fabsf(q[4]), m5 = fabsf(q[5]);
+        float p03 = amin_t(m0, m3), p14 = amin_t(m1, m4), p25 = amin_t(m2, m5);
+        float o0 = amin_t(p14, p25), o1 = amin_t(p03, p25), o2 = amin_t(p03, p14);
+        r[0] = __fmul_rn(amin_t(o0, m3), beta);
+        r[3] = __fmul_rn(amin_t(o0, m0), beta);
+        r[1] = __fmul_rn(amin_t(o1, m4), beta);
+        r[4] = __fmul_rn(amin_t(o1, m1), beta);
+        r[2] = __fmul_rn(amin_t(o2, m5), beta);
+        r[5] = __fmul_rn(amin_t(o2, m2), beta);
+    } else {
+        static_assert(D == 3, "Profile T defines degree 3 and 6 checks");
+        float m0 = fabsf(q[0]), m1 = fabsf(q[1]), m2 = fabsf(q[2]);
+        r[0] = __fmul_rn(amin_t(m1, m2), beta);
+        r[1] = __fmul_rn(amin_t(m0, m2), beta);
+        r[2] = __fmul_rn(amin_t(m0, m1), beta);
+    }
+}
+
+// One layer: lane updates checks j = lane and j = lane + 64.
+template <int LI>
+__device__ __forceinline__ void run_layer(uint16_t *Ls, uint32_t (&R)[NR], int lane, float beta6, float beta3) {
+    constexpr ar4ja::Layer Ly = S12.layer[LI];
+    constexpr int D = Ly.deg;
+    constexpr int R0 = r_offset(LI);
+    const float beta = (D == 6) ? beta6 : beta3;
+    // Opaque copy of the lane id: keeps the shifted LDS addresses from being
+    // hoisted out of the iteration loop (they would pin ~60 VGPRs).
+    asm volatile("" : "+v"(lane));
+    int ia[D], ib[D];
+    float qa[D], qb[D];
+    uint32_t sa = 0, sb = 0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        const int base = Ly.slot[d].base, sh = Ly.slot[d].shift;
+        int xa = sh == 0 ? lane : ((lane + sh) & (Z - 1));
+        int xb = sh == 0 ? lane + 64 : (xa ^ 64);
+        ia[d] = base + xa;
+        ib[d] = base + xb;
+        const uint32_t rp = R[R0 + d];
+        qa[d] = u2f(rne_bits(__fsub_rn(ld_bf(Ls + ia[d]), u2f(rp << 16))));
+        qb[d] = u2f(rne_bits(__fsub_rn(ld_bf(Ls + ib[d]), u2f(rp & 0xFFFF0000u))));
+        sa ^= f2u(qa[d]);
+        sb ^= f2u(qb[d]);
+    }
+    float ra[D], rb[D];
+    check_tree<D>(qa, beta, ra);
+    check_tree<D>(qb, beta, rb);
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        uint32_t oa = rne_bits(ra[d]) | ((sa ^ f2u(qa[d])) & 0x80000000u);
+        uint32_t ob = rne_bits(rb[d]) | ((sb ^ f2u(qb[d])) & 0x80000000u);
+        uint32_t la = rne_bits(__fadd_rn(qa[d], u2f(oa)));
+        uint32_t lb = rne_bits(__fadd_rn(qb[d], u2f(ob)));
+        Ls[ia[d]] = (uint16_t)(la >> 16);
+        Ls[ib[d]] = (uint16_t)(lb >> 16);
+        R[R0 + d] = (ob & 0xFFFF0000u) | (oa >> 16);
+    }
+    wave_lds_fence();
+}
+
+template <int... LI>
+__device__ __forceinline__ void run_iteration(uint16_t *Ls, uint32_t (&R)[NR], int lane, float beta6, float beta3,
+                                              std::integer_sequence<int, LI...>) {
+    (run_layer<LI>(Ls, R, lane, beta6, beta3), ...);
+}
+
+// 128-bit rotate right by a compile-time amount: bit j of the result is bit
+// (j + s) mod 128 of (hi:lo).
+template <int s>
+__device__ __forceinline__ void rotr128(uint64_t lo, uint64_t hi, uint64_t &olo, uint64_t &ohi) {
+    if constexpr (s == 0) {
+        olo = lo;
+        ohi = hi;
+    } else if constexpr (s < 64) {
+        olo = (lo >> s) | (hi << (64 - s));
+        ohi = (hi >> s) | (lo << (64 - s));
+    } else if constexpr (s == 64) {
+        olo = hi;
+        ohi = lo;
+    } else {
+        olo = (hi >> (s - 64)) | (lo << (128 - s));
+        ohi = (lo >> (s - 64)) | (hi << (128 - s));
+    }
+}
+
+// Hard decision "bf16 < 0.0f" (stepper:781): sign set and magnitude nonzero.
+__device__ __forceinline__ bool hard_neg(uint16_t h) { return h > 0x8000u; }
+
+// Parity of the layer's 128 checks from the hard decisions in Ls.  Each edge
+// slot contributes the 128-bit hard-decision mask of its variable quarter
+// (two wave ballots), rotated by the slot's shift; the XOR over slots is the
+// layer's syndrome vector.  Masks are formed per slot so that only a few
+// 64-bit scalars are live at a time.
+template <int LI, int... D>
+__device__ __forceinline__ uint64_t layer_syndrome(const uint16_t *Ls, int lane, std::integer_sequence<int, D...>) {
+    constexpr ar4ja::Layer Ly = S12.layer[LI];
+    uint64_t plo = 0, phi = 0;
+    (
+        [&] {
+            constexpr int base = Ly.slot[D].base;
+            const uint64_t lo = __ballot(hard_neg(Ls[base + lane]));
+            const uint64_t hi = __ballot(hard_neg(Ls[base + 64 + lane]));
+            uint64_t a, b;
+            rotr128<Ly.slot[D].shift>(lo, hi, a, b);
+            plo ^= a;
+            phi ^= b;
+        }(),
+        ...);
+    return plo | phi;
+}
+
+// Layers are tested one at a time and the test stops at the first layer with
+// an unsatisfied check, so an iteration that has not converged usually costs
+// one layer's ballots.  The memory clobber keeps the compiler from caching
+// ballots across layers (which would pin dozens of scalar registers).
+template <int LI>
+__device__ __forceinline__ bool syndrome_from(const uint16_t *Ls, int lane) {
+    if constexpr (LI == NLAYERS) {
+        return true;
+    } else {
+        asm volatile("" ::: "memory");
+        if (layer_syndrome<LI>(Ls, lane, std::make_integer_sequence<int, S12.layer[LI].deg>{}) != 0) return false;
+        return syndrome_from<LI + 1>(Ls, lane);
+    }
+}
+
+__device__ __forceinline__ bool wave_syndrome_ok(const uint16_t *Ls, int lane) { return syndrome_from<0>(Ls, lane); }
+
+// Decode the codeword held in Ls.  Returns the iteration of convergence
+// (1..max_iter) or 0 if the syndrome never cleared within max_iter.
+__device__ __forceinline__ int decode_wave(uint16_t *Ls, int lane, int max_iter, float beta6, float beta3) {
+    uint32_t R[NR];
+#pragma unroll
+    for (int i = 0; i < NR; ++i) R[i] = 0u;  // bf16 +0: iteration 0 uses q = L
+    wave_lds_fence();
+    for (int it = 0; it < max_iter; ++it) {
+        run_iteration(Ls, R, lane, beta6, beta3, std::make_integer_sequence<int, NLAYERS>{});
+        if (wave_syndrome_ok(Ls, lane)) return it + 1;
+    }
+    return 0;
+}
+
+// Bit errors of an unconverged codeword: info bits k < K with bf16 <= 0.0f
+// (stepper:801-809).
+__device__ __forceinline__ uint32_t wave_info_errors(const uint16_t *Ls, int lane) {
+    uint32_t be = 0;
+#pragma unroll
+    for (int t = 0; t < KINFO / 64; ++t) {
+        uint16_t h = Ls[t * 64 + lane];
+        be += (uint32_t)__popcll(__ballot((h & 0x8000u) || h == 0));
+    }
+    return be;
+}
+
+// Channel LLRs of codeword w into Ls: Philox(ctr = (w_lo, w_hi, blk, 0),
+// key = seed) -> 4 normals -> RNE_bf16(mu + sigma z); punctured = +0.
+__device__ __forceinline__ void wave_gen_channel(uint16_t *Ls, int lane, uint64_t seed, uint64_t w, float mu,
+                                                 float sigma) {
+    const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
+#pragma unroll 2
+    for (int t = 0; t < NTX / 4 / 64; ++t) {
+        const uint32_t blk = (uint32_t)(t * 64 + lane);
+        u32x4 x = philox4x32_10((uint32_t)w, (uint32_t)(w >> 32), blk, 0u, k0, k1);
+        float z0, z1, z2, z3;
+        box_muller(x.x, x.y, z0, z1);
+        box_muller(x.z, x.w, z2, z3);
+        uint32_t lo = (llr_bits(mu, sigma, z1) & 0xFFFF0000u) | (llr_bits(mu, sigma, z0) >> 16);
+        uint32_t hi = (llr_bits(mu, sigma, z3) & 0xFFFF0000u) | (llr_bits(mu, sigma, z2) >> 16);
+        *reinterpret_cast<uint2 *>(Ls + 4 * blk) = make_uint2(lo, hi);
+    }
+    // punctured variables NTX..NV-1: 512 x bf16 +0 = 64 lanes x 16 B
+    *reinterpret_cast<uint4 *>(Ls + NTX + 8 * lane) = make_uint4(0u, 0u, 0u, 0u);
+}
+
+__device__ __forceinline__ uint32_t next_codeword(uint32_t *queue, int lane) {
+    uint32_t v = 0;
+    if (lane == 0) v = atomicAdd(queue, 1u);
+    return __builtin_amdgcn_readfirstlane(v);
+}
+
+__global__ void __launch_bounds__(64 * WAVES_PER_WG)
+    k_decode_llrs(const uint16_t *__restrict__ llr_in, uint16_t *__restrict__ llr_out, uint32_t *__restrict__ iters,
+                  uint8_t *__restrict__ ok, uint32_t n_cw, int max_iter, float beta6, float beta3, uint32_t *queue) {
+    __shared__ __attribute__((aligned(16))) uint16_t smem[WAVES_PER_WG][NV];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    uint16_t *Ls = smem[wv];
+    for (;;) {
+        const uint32_t cw = next_codeword(queue, lane);
+        if (cw >= n_cw) break;
+        const uint4 *src = reinterpret_cast<const uint4 *>(llr_in + (size_t)cw * NV);
+#pragma unroll
+        for (int i = 0; i < NV / 8 / 64; ++i) reinterpret_cast<uint4 *>(Ls)[i * 64 + lane] = src[i * 64 + lane];
+        const int it = decode_wave(Ls, lane, max_iter, beta6, beta3);
+        wave_lds_fence();
+        uint4 *dst = reinterpret_cast<uint4 *>(llr_out + (size_t)cw * NV);
+#pragma unroll
+        for (int i = 0; i < NV / 8 / 64; ++i) dst[i * 64 + lane] = reinterpret_cast<const uint4 *>(Ls)[i * 64 + lane];
+        if (lane == 0) {
+            iters[cw] = it ? (uint32_t)it : (uint32_t)max_iter;
+            ok[cw] = it ? 1 : 0;
+        }
+        wave_lds_fence();
+    }
+}
+
+// Fused Monte Carlo: channel generation -> decode -> tallies.
+// counters: [0] blocks, [1] frame errors, [2] bit errors, [3] iteration sum;
+// hist[i] (i = 1..max_iter) counts convergence at iteration i, hist[0] failures.
+__global__ void __launch_bounds__(64 * WAVES_PER_WG)
+    k_monte_carlo(uint64_t seed, uint64_t first_cw, uint32_t n_cw, float mu, float sigma, int max_iter, float beta6,
+                  float beta3, uint32_t *queue, unsigned long long *counters, unsigned long long *hist) {
+    __shared__ __attribute__((aligned(16))) uint16_t smem[WAVES_PER_WG][NV];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    uint16_t *Ls = smem[wv];
+    uint64_t blocks = 0, fe = 0, be = 0, its = 0;
+    for (;;) {
+        const uint32_t cw = next_codeword(queue, lane);
+        if (cw >= n_cw) break;
+        wave_gen_channel(Ls, lane, seed, first_cw + cw, mu, sigma);
+        wave_lds_fence();
+        const int it = decode_wave(Ls, lane, max_iter, beta6, beta3);
+        blocks += 1;
+        if (it) {
+            its += (uint64_t)it;
+        } else {
+            its += (uint64_t)max_iter;
+            fe += 1;
+            be += wave_info_errors(Ls, lane);
+        }
+        if (lane == 0) atomicAdd(hist + it, 1ull);
+        wave_lds_fence();
+    }
+    if (lane == 0 && blocks) {
+        atomicAdd(counters + 0, (unsigned long long)blocks);
+        if (fe) atomicAdd(counters + 1, (unsigned long long)fe);
+        if (be) atomicAdd(counters + 2, (unsigned long long)be);
+        atomicAdd(counters + 3, (unsigned long long)its);
+    }
+}
+
+__global__ void __launch_bounds__(256)
+    k_gen_channel(uint64_t seed, uint64_t first_cw, uint32_t n_cw, float mu, float sigma, uint16_t *__restrict__ out) {
+    // one thread per Philox block of 4 symbols
+    const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t blocks_per_cw = NTX / 4;
+    if (gid >= (uint64_t)n_cw * blocks_per_cw) return;
+    const uint32_t c = (uint32_t)(gid / blocks_per_cw), blk = (uint32_t)(gid % blocks_per_cw);
+    const uint64_t w = first_cw + c;
+    u32x4 x = philox4x32_10((uint32_t)w, (uint32_t)(w >> 32), blk, 0u, (uint32_t)seed, (uint32_t)(seed >> 32));
+    float z0, z1, z2, z3;
+    box_muller(x.x, x.y, z0, z1);
+    box_muller(x.z, x.w, z2, z3);
+    uint32_t lo = (llr_bits(mu, sigma, z1) & 0xFFFF0000u) | (llr_bits(mu, sigma, z0) >> 16);
+    uint32_t hi = (llr_bits(mu, sigma, z3) & 0xFFFF0000u) | (llr_bits(mu, sigma, z2) >> 16);
+    uint16_t *L = out + (size_t)c * NV;
+    *reinterpret_cast<uint2 *>(L + 4 * blk) = make_uint2(lo, hi);
+    if (blk < (NV - NTX) / 4) *reinterpret_cast<uint2 *>(L + NTX + 4 * blk) = make_uint2(0u, 0u);
+}
+
+__global__ void k_fill_u64(unsigned long long *p, size_t n, unsigned long long v) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void k_fill_u32(uint32_t *p, size_t n, uint32_t v) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Launchers (host side)
+// ---------------------------------------------------------------------------
+int t12_waves_per_wg() { return WAVES_PER_WG; }
+
+hipError_t t12_occupancy(int *wg_per_cu) {
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(wg_per_cu, k_monte_carlo, 64 * WAVES_PER_WG, 0);
+}
+
+hipError_t t12_occupancy_decode(int *wg_per_cu) {
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(wg_per_cu, k_decode_llrs, 64 * WAVES_PER_WG, 0);
+}
+
+hipError_t launch_fill_u64(unsigned long long *p, size_t n, unsigned long long v, hipStream_t st) {
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_u64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_u32(uint32_t *p, size_t n, uint32_t v, hipStream_t st) {
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n, v);
+    return hipGetLastError();
+}
+
+hipError_t t12_launch_decode(const uint16_t *llr_in, uint16_t *llr_out, uint32_t *iters, uint8_t *ok, uint32_t n_cw,
+                             int max_iter, float beta, uint32_t *queue, int grid, hipStream_t st) {
+    const float beta3 = beta < 1.0f ? beta : 1.0f;  // stepper:300-304
+    hipLaunchKernelGGL(k_decode_llrs, dim3(grid), dim3(64 * WAVES_PER_WG), 0, st, llr_in, llr_out, iters, ok, n_cw,
+                       max_iter, beta, beta3, queue);
+    return hipGetLastError();
+}
+
+hipError_t t12_launch_mc(uint64_t seed, uint64_t first_cw, uint32_t n_cw, float mu, float sigma, int max_iter,
+                         float beta, uint32_t *queue, unsigned long long *counters, unsigned long long *hist, int grid,
+                         hipStream_t st) {
+    const float beta3 = beta < 1.0f ? beta : 1.0f;
+    hipLaunchKernelGGL(k_monte_carlo, dim3(grid), dim3(64 * WAVES_PER_WG), 0, st, seed, first_cw, n_cw, mu, sigma,
+                       max_iter, beta, beta3, queue, counters, hist);
+    return hipGetLastError();
+}
+
+hipError_t t12_launch_gen(uint64_t seed, uint64_t first_cw, uint32_t n_cw, float mu, float sigma, uint16_t *out,
+                          hipStream_t st) {
+    const uint64_t threads = (uint64_t)n_cw * (NTX / 4);
+    if (!threads) return hipSuccess;
+    hipLaunchKernelGGL(k_gen_channel, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, st, seed, first_cw, n_cw,
+                       mu, sigma, out);
+    return hipGetLastError();
+}
+
+// Compiled-in schedule, for host-side cross-checks: per layer
+// [first_row, deg, (base, shift) x deg].
+int t12_schedule(int32_t *out, int cap) {
+    int n = 0;
+    for (int li = 0; li < NLAYERS; ++li) {
+        const ar4ja::Layer &L = S12.layer[li];
+        if (n + 2 + 2 * L.deg > cap) return -1;
+        out[n++] = L.first_row;
+        out[n++] = L.deg;
+        for (int d = 0; d < L.deg; ++d) {
+            out[n++] = L.slot[d].base;
+            out[n++] = L.slot[d].shift;
+        }
+    }
+    return n;
+}
+
+}  // namespace ldpc

@@ -252,7 +252,14 @@ BLOCK_ROW_ORDER_T = (1, 2, 0)
 
 def ar4ja(K: int = 1024, rate: str = "1/2", block_row_order: Sequence[int] = BLOCK_ROW_ORDER_T) -> Code:
     """Build the CCSDS AR4JA code (K, rate) with its quarter-layer schedule."""
-    M = block_size(K, rate)
+    return ar4ja_m(block_size(K, rate), rate, block_row_order)
+
+
+def ar4ja_m(M: int, rate: str = "1/2", block_row_order: Sequence[int] = BLOCK_ROW_ORDER_T) -> Code:
+    """AR4JA parity-check structure of a rate for submatrix size M (a power of
+    two, 128..8192); K follows as M * (number of block columns - 3)."""
+    if M < 128 or M > 8192 or M & (M - 1):
+        raise ValueError(f"submatrix size M={M} is not one of 128, 256, ..., 8192")
     Z = M // 4
     terms = _block_terms(rate)
     n_blk_cols = max(c for _, c, _ in terms) + 1
@@ -283,6 +290,7 @@ def ar4ja(K: int = 1024, rate: str = "1/2", block_row_order: Sequence[int] = BLO
             bases = [e.base for e in edges]
             assert len(set(bases)) == len(bases)
             layers.append((b * M + j * Z, edges))
+    K = N - 3 * M
     return Code(name=f"AR4JA r{rate} K={K}", n=N, m=3 * M, k=K, n_punct=M, z=Z, rows=rows, layers=layers)
 
 

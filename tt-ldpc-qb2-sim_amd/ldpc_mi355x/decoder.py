@@ -1,0 +1,174 @@
+"""Host-side API over libldpc: a Monte Carlo LDPC engine on one GPU.
+
+`Engine` is the in-process counterpart of the reference's engine binaries
+(engines/scalar_5w/ldpc_sim_scalar.cpp, engines/circulant_dma_4cw/
+ldpc_sim_circulant_dma.cpp) and of the golden stepper's dump replay
+(tests/test_circulant_dma_stepper.cpp:861-990).  All decoding runs in the HIP
+kernels of libldpc; this module only marshals arguments.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import dataclasses
+from typing import Optional
+
+import numpy as np
+
+from . import _lib as L
+
+RATE_FRACTIONS = {"1/2": (1, 2), "2/3": (2, 3), "4/5": (4, 5)}
+
+
+@dataclasses.dataclass
+class PointResult:
+    ebn0_db: float
+    blocks: int
+    frame_errors: int
+    bit_errors: int
+    iter_sum: int
+    next_cw: int
+    elapsed_s: float
+    kernel_s: float
+    launches: int
+    hist: np.ndarray  # hist[i]: converged at iteration i (i >= 1); hist[0]: not converged
+    n_tx: int
+    k_info: int
+
+    @property
+    def fer(self) -> float:
+        return self.frame_errors / self.blocks if self.blocks else 0.0
+
+    @property
+    def ber(self) -> float:
+        """Bit errors per information bit (stepper:824)."""
+        return self.bit_errors / (self.blocks * self.k_info) if self.blocks else 0.0
+
+    @property
+    def mean_iterations(self) -> float:
+        return self.iter_sum / self.blocks if self.blocks else 0.0
+
+    @property
+    def gsps(self) -> float:
+        """Coded channel symbols per second over the call's wall time, in G."""
+        return self.blocks * self.n_tx / self.elapsed_s / 1e9 if self.elapsed_s else 0.0
+
+    @property
+    def kernel_gsps(self) -> float:
+        return self.blocks * self.n_tx / self.kernel_s / 1e9 if self.kernel_s else 0.0
+
+
+def compiled_schedule(k_info: int = 1024, rate: str = "1/2"):
+    """Layer schedule compiled into the kernels: [(first_row, [(base, shift), ...]), ...]."""
+    lib = L.load()
+    num, den = RATE_FRACTIONS[rate]
+    desc = L.CodeDesc(L.LDPC_CODE_AR4JA, k_info, num, den)
+    buf = np.zeros(8192, dtype=np.int32)
+    n = lib.ldpc_get_schedule(C.byref(desc), buf.ctypes.data, buf.size)
+    if n < 0:
+        raise L.LdpcError(n, "no compiled schedule for this code")
+    out, i = [], 0
+    while i < n:
+        first, deg = int(buf[i]), int(buf[i + 1])
+        out.append((first, [(int(buf[i + 2 + 2 * d]), int(buf[i + 3 + 2 * d])) for d in range(deg)]))
+        i += 2 + 2 * deg
+    return out
+
+
+class Engine:
+    """One libldpc context: a code, a decoder configuration and a device."""
+
+    def __init__(self, k_info: int = 1024, rate: str = "1/2", max_iter: int = 200, beta: float = 0.9375,
+                 profile: int = L.LDPC_PROFILE_T, device: int = 0):
+        self._lib = L.load()
+        num, den = RATE_FRACTIONS[rate]
+        desc = L.CodeDesc(L.LDPC_CODE_AR4JA, k_info, num, den)
+        cfg = L.DecoderCfg(profile, max_iter, beta)
+        h = C.c_void_p()
+        rc = self._lib.ldpc_create(C.byref(desc), C.byref(cfg), device, C.byref(h))
+        if rc != L.LDPC_OK:
+            raise L.LdpcError(rc, self._lib.ldpc_last_error(None).decode())
+        self._h = h
+        info = L.CodeInfo()
+        self._check(self._lib.ldpc_get_code_info(self._h, C.byref(info)))
+        self.n, self.m, self.k_info, self.n_tx = info.n, info.m, info.k_info, info.n_tx
+        self.z, self.n_layers, self.n_edges = info.z, info.n_layers, info.n_edges
+        self.max_iter, self.beta, self.rate = max_iter, beta, rate
+
+    def _check(self, rc: int) -> None:
+        if rc != L.LDPC_OK:
+            raise L.LdpcError(rc, self._lib.ldpc_last_error(self._h).decode())
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.ldpc_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- queries -----------------------------------------------------------
+    def channel_params(self, ebn0_db: float):
+        mu, sg = C.c_float(), C.c_float()
+        self._check(self._lib.ldpc_channel_params(self._h, ebn0_db, C.byref(mu), C.byref(sg)))
+        return mu.value, sg.value
+
+    def schedule(self):
+        return compiled_schedule(self.k_info, self.rate)
+
+    @property
+    def stream(self) -> int:
+        return self._lib.ldpc_stream(self._h) or 0
+
+    # -- work --------------------------------------------------------------
+    def run_point(self, ebn0_db: float, max_blocks: int, target_fe: int = 0, seed: int = 0x5EED2026,
+                  first_cw: int = 0, chunk_cw: int = 1 << 20, result: Optional[PointResult] = None) -> PointResult:
+        """Monte Carlo at one Eb/N0; accumulates into `result` if given."""
+        pt = L.PointCfg(ebn0_db, seed, first_cw, max_blocks, target_fe, chunk_cw)
+        res = L.PointResult()
+        hist = np.zeros(self.max_iter + 1, dtype=np.uint64)
+        self._check(self._lib.ldpc_run_point(self._h, C.byref(pt), C.byref(res), hist.ctypes.data))
+        out = PointResult(ebn0_db, res.blocks, res.frame_errors, res.bit_errors, res.iter_sum, res.next_cw,
+                          res.elapsed_ns * 1e-9, res.kernel_ns * 1e-9, res.launches, hist, self.n_tx, self.k_info)
+        if result is not None:
+            result.blocks += out.blocks
+            result.frame_errors += out.frame_errors
+            result.bit_errors += out.bit_errors
+            result.iter_sum += out.iter_sum
+            result.next_cw = out.next_cw
+            result.elapsed_s += out.elapsed_s
+            result.kernel_s += out.kernel_s
+            result.launches += out.launches
+            result.hist = result.hist + out.hist
+            return result
+        return out
+
+    def decode_llrs(self, llr: np.ndarray):
+        """Decode bf16 channel LLRs [n_cw, N] -> (final LLRs, iterations, converged)."""
+        llr = np.ascontiguousarray(llr, dtype=np.uint16).reshape(-1, self.n)
+        n_cw = llr.shape[0]
+        out = np.empty_like(llr)
+        iters = np.empty(n_cw, dtype=np.uint32)
+        ok = np.empty(n_cw, dtype=np.uint8)
+        self._check(self._lib.ldpc_decode_llrs(self._h, llr.ctypes.data, n_cw, out.ctypes.data, iters.ctypes.data,
+                                               ok.ctypes.data))
+        return out, iters, ok
+
+    def decode_llrs_device_async(self, d_llr: int, n_cw: int, d_out: int, d_iters: int, d_ok: int) -> None:
+        self._check(self._lib.ldpc_decode_llrs_device_async(self._h, d_llr, n_cw, d_out, d_iters, d_ok))
+
+    def sync(self) -> None:
+        self._check(self._lib.ldpc_sync(self._h))
+
+    def gen_channel(self, seed: int, first_cw: int, n_cw: int, ebn0_db: float) -> np.ndarray:
+        out = np.empty((n_cw, self.n), dtype=np.uint16)
+        self._check(self._lib.ldpc_gen_channel(self._h, seed, first_cw, n_cw, ebn0_db, out.ctypes.data))
+        return out

@@ -38,9 +38,12 @@ N_TX = 2048
 # bytes (10 B per edge: L read/write, R read/write, syndrome read).
 OPS_PER_CWI = 1024 * 239 + 512 * 86
 LDS_BYTES_PER_CWI = 7680 * 10
-# MI355X: 256 CUs x 4 SIMD x 16 lanes x 2.4 GHz (non-packed VALU);
+# MI355X: 256 CUs x 128 VALU lanes/clk x 2.4 GHz = 78.6 T lane-op/s, the
+# published fp32 vector peak (157.3 TFLOP/s) counted per lane-op.  Measured on
+# this kernel: 1.29 wave64 VALU instructions per CU-cycle issue without packed
+# math (SQ_INSTS_VALU / GRBM_GUI_ACTIVE), so a 64-lane/clk peak is too low.
 # LDS 128 B/clk/CU; HBM3E 8 TB/s.
-PEAK_VALU_TOPS = 256 * 4 * 16 * 2.4e9 / 1e12
+PEAK_VALU_TOPS = 256 * 128 * 2.4e9 / 1e12
 PEAK_LDS_TBS = 256 * 128 * 2.4e9 / 1e12
 
 
@@ -110,6 +113,23 @@ def read_traffic():
     return float(m.group(1)) if m else None
 
 
+def copy_calibration(mib: int, reps: int = 3) -> None:
+    """Device-to-device copies of a known size through the HIP runtime libldpc
+    links, for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE readings."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so.7")
+    n = mib << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(n)) == 0 and hip.hipMalloc(C.byref(b), C.c_size_t(n)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(n)) == 0
+    for _ in range(reps):
+        assert hip.hipMemcpy(b, a, C.c_size_t(n), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"copy_calibration_bytes_per_copy": n, "copies": reps}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -117,7 +137,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run device-to-device copies of MIB MiB (counter calibration)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build measures one GPU only (the decode path shards by codeword id across GPUs "
               "but no multi-GPU figure is produced); run with --gpus 1", file=sys.stderr)

@@ -129,18 +129,37 @@ def test_monte_carlo_full_launch_invariants(engine):
 
 
 def test_device_pointer_entry_point(engine, oracle, code12):
-    """ldpc_decode_llrs_device_async on torch-owned device buffers."""
-    import torch
+    """ldpc_decode_llrs_device_async on device buffers allocated with the same
+    HIP runtime libldpc links (a second runtime in one process, such as the
+    one bundled with torch, cannot open the GPU once this one has)."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so.7")
+    H2D, D2H = 1, 2
+
+    def dmalloc(nbytes):
+        p = C.c_void_p()
+        assert hip.hipMalloc(C.byref(p), C.c_size_t(nbytes)) == 0
+        return p
+
     mu, sg = channel_params(1.0)
-    llr = oracle.gen_channel(code12, 77, 0, 200, mu, sg)
-    d_in = torch.from_numpy(llr.view(np.int16)).cuda()
-    d_out = torch.empty_like(d_in)
-    d_it = torch.empty(200, dtype=torch.int32, device="cuda")
-    d_ok = torch.empty(200, dtype=torch.uint8, device="cuda")
-    torch.cuda.synchronize()
-    _print_case("device async decode n=200")
-    engine.decode_llrs_device_async(d_in.data_ptr(), 200, d_out.data_ptr(), d_it.data_ptr(), d_ok.data_ptr())
-    engine.sync()
+    n = 200
+    llr = oracle.gen_channel(code12, 77, 0, n, mu, sg)
+    d_in, d_out = dmalloc(llr.nbytes), dmalloc(llr.nbytes)
+    d_it, d_ok = dmalloc(4 * n), dmalloc(n)
+    try:
+        assert hip.hipMemcpy(d_in, llr.ctypes.data_as(C.c_void_p), C.c_size_t(llr.nbytes), H2D) == 0
+        _print_case("device async decode n=200")
+        engine.decode_llrs_device_async(d_in.value, n, d_out.value, d_it.value, d_ok.value)
+        engine.sync()
+        out = np.empty_like(llr)
+        it = np.empty(n, dtype=np.uint32)
+        ok = np.empty(n, dtype=np.uint8)
+        for host, dev in ((out, d_out), (it, d_it), (ok, d_ok)):
+            assert hip.hipMemcpy(host.ctypes.data_as(C.c_void_p), dev, C.c_size_t(host.nbytes), D2H) == 0
+    finally:
+        for p in (d_in, d_out, d_it, d_ok):
+            hip.hipFree(p)
     o_out, o_it, o_ok = oracle.decode_t(code12, llr, 200, 0.9375)
-    np.testing.assert_array_equal(d_out.cpu().numpy().view(np.uint16), o_out)
-    np.testing.assert_array_equal(d_it.cpu().numpy().view(np.uint32), o_it)
+    np.testing.assert_array_equal(out, o_out)
+    np.testing.assert_array_equal(it, o_it)
+    np.testing.assert_array_equal(ok, o_ok)

@@ -86,7 +86,22 @@ __device__ __forceinline__ void check_tree(const float (&q)[D], float beta, floa
     }
 }
 
-// One layer: lane updates checks j = lane and j = lane + 64.
+// Two fp32 values -> packed bf16 pair (lo = a, hi = b), round to nearest
+// even.  v_cvt_pk_bf16_f32 was checked exhaustively against the integer
+// formula (u + 0x7FFF + lsb) >> 16 on every non-NaN fp32 pattern (2^32 - 2^24
+// + 2 inputs, 0 mismatches, MI355X); NaN never reaches it here.
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+typedef float f32x2_t __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint32_t cvt_pk_bf16(float a, float b) {
+    const bf16x2_t r = __builtin_convertvector((f32x2_t){a, b}, bf16x2_t);
+    return __builtin_bit_cast(uint32_t, r);
+}
+__device__ __forceinline__ float lo_f(uint32_t p) { return u2f(p << 16); }
+__device__ __forceinline__ float hi_f(uint32_t p) { return u2f(p & 0xFFFF0000u); }
+
+// One layer: lane updates checks j = lane (a) and j = lane + 64 (b).  q, R'
+// and L' of the two checks travel as packed bf16 pairs; the signs of R' are
+// applied to the packed word (RNE commutes with the sign bit).
 template <int LI>
 __device__ __forceinline__ void run_layer(uint16_t *Ls, uint32_t (&R)[NR], int lane, float beta6, float beta3) {
     constexpr ar4ja::Layer Ly = S12.layer[LI];
@@ -97,8 +112,9 @@ __device__ __forceinline__ void run_layer(uint16_t *Ls, uint32_t (&R)[NR], int l
     // hoisted out of the iteration loop (they would pin ~60 VGPRs).
     asm volatile("" : "+v"(lane));
     int ia[D], ib[D];
+    uint32_t qp[D];
     float qa[D], qb[D];
-    uint32_t sa = 0, sb = 0;
+    uint32_t sp = 0;  // sign parity of both checks: bit 15 (a), bit 31 (b)
 #pragma unroll
     for (int d = 0; d < D; ++d) {
         const int base = Ly.slot[d].base, sh = Ly.slot[d].shift;
@@ -107,23 +123,21 @@ __device__ __forceinline__ void run_layer(uint16_t *Ls, uint32_t (&R)[NR], int l
         ia[d] = base + xa;
         ib[d] = base + xb;
         const uint32_t rp = R[R0 + d];
-        qa[d] = u2f(rne_bits(__fsub_rn(ld_bf(Ls + ia[d]), u2f(rp << 16))));
-        qb[d] = u2f(rne_bits(__fsub_rn(ld_bf(Ls + ib[d]), u2f(rp & 0xFFFF0000u))));
-        sa ^= f2u(qa[d]);
-        sb ^= f2u(qb[d]);
+        qp[d] = cvt_pk_bf16(__fsub_rn(ld_bf(Ls + ia[d]), lo_f(rp)), __fsub_rn(ld_bf(Ls + ib[d]), hi_f(rp)));
+        qa[d] = lo_f(qp[d]);
+        qb[d] = hi_f(qp[d]);
+        sp ^= qp[d];
     }
     float ra[D], rb[D];
     check_tree<D>(qa, beta, ra);
     check_tree<D>(qb, beta, rb);
 #pragma unroll
     for (int d = 0; d < D; ++d) {
-        uint32_t oa = rne_bits(ra[d]) | ((sa ^ f2u(qa[d])) & 0x80000000u);
-        uint32_t ob = rne_bits(rb[d]) | ((sb ^ f2u(qb[d])) & 0x80000000u);
-        uint32_t la = rne_bits(__fadd_rn(qa[d], u2f(oa)));
-        uint32_t lb = rne_bits(__fadd_rn(qb[d], u2f(ob)));
-        Ls[ia[d]] = (uint16_t)(la >> 16);
-        Ls[ib[d]] = (uint16_t)(lb >> 16);
-        R[R0 + d] = (ob & 0xFFFF0000u) | (oa >> 16);
+        const uint32_t rn = cvt_pk_bf16(ra[d], rb[d]) | ((sp ^ qp[d]) & 0x80008000u);
+        const uint32_t ln = cvt_pk_bf16(__fadd_rn(qa[d], lo_f(rn)), __fadd_rn(qb[d], hi_f(rn)));
+        Ls[ia[d]] = (uint16_t)ln;
+        Ls[ib[d]] = (uint16_t)(ln >> 16);
+        R[R0 + d] = rn;
     }
     wave_lds_fence();
 }
@@ -242,10 +256,35 @@ __device__ __forceinline__ void wave_gen_channel(uint16_t *Ls, int lane, uint64_
     *reinterpret_cast<uint4 *>(Ls + NTX + 8 * lane) = make_uint4(0u, 0u, 0u, 0u);
 }
 
-__device__ __forceinline__ uint32_t next_codeword(uint32_t *queue, int lane) {
+// Work split of a launch: codewords [0, n_static) are dealt round-robin to the
+// waves (no atomics); the rest are handed out one at a time from a device
+// queue so that waves that drew easy codewords absorb the imbalance of the
+// tail.  n_static is 7/8 of the launch rounded down to whole rounds.
+struct WorkSplit {
+    uint32_t n_cw, n_static, n_waves, wave;
+    uint32_t k;  // next static round of this wave
+};
+
+__device__ __forceinline__ WorkSplit work_split(uint32_t n_cw) {
+    WorkSplit w;
+    w.n_cw = n_cw;
+    w.n_waves = gridDim.x * WAVES_PER_WG;
+    w.wave = blockIdx.x * WAVES_PER_WG + (threadIdx.x >> 6);
+    w.n_static = (uint32_t)(((uint64_t)n_cw * 7u / 8u) / w.n_waves * w.n_waves);
+    w.k = 0;
+    return w;
+}
+
+__device__ __forceinline__ uint32_t next_codeword(WorkSplit &w, uint32_t *queue, int lane) {
+    const uint32_t s = w.wave + w.k * w.n_waves;
+    if (s < w.n_static) {
+        ++w.k;
+        return s;
+    }
     uint32_t v = 0;
     if (lane == 0) v = atomicAdd(queue, 1u);
-    return __builtin_amdgcn_readfirstlane(v);
+    v = __builtin_amdgcn_readfirstlane(v);
+    return v >= w.n_cw - w.n_static ? w.n_cw : w.n_static + v;
 }
 
 __global__ void __launch_bounds__(64 * WAVES_PER_WG)
@@ -254,8 +293,9 @@ __global__ void __launch_bounds__(64 * WAVES_PER_WG)
     __shared__ __attribute__((aligned(16))) uint16_t smem[WAVES_PER_WG][NV];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     uint16_t *Ls = smem[wv];
+    WorkSplit ws = work_split(n_cw);
     for (;;) {
-        const uint32_t cw = next_codeword(queue, lane);
+        const uint32_t cw = next_codeword(ws, queue, lane);
         if (cw >= n_cw) break;
         const uint4 *src = reinterpret_cast<const uint4 *>(llr_in + (size_t)cw * NV);
 #pragma unroll
@@ -276,15 +316,25 @@ __global__ void __launch_bounds__(64 * WAVES_PER_WG)
 // Fused Monte Carlo: channel generation -> decode -> tallies.
 // counters: [0] blocks, [1] frame errors, [2] bit errors, [3] iteration sum;
 // hist[i] (i = 1..max_iter) counts convergence at iteration i, hist[0] failures.
+// The histogram is binned per workgroup in LDS (bins 0..HIST_LDS-1) and
+// flushed once per workgroup at the end; iteration counts beyond the LDS bins
+// go straight to global memory.  One global atomic per codeword would cost a
+// 64-byte HBM write each.
+constexpr int HIST_LDS = 256;
+
 __global__ void __launch_bounds__(64 * WAVES_PER_WG)
     k_monte_carlo(uint64_t seed, uint64_t first_cw, uint32_t n_cw, float mu, float sigma, int max_iter, float beta6,
                   float beta3, uint32_t *queue, unsigned long long *counters, unsigned long long *hist) {
     __shared__ __attribute__((aligned(16))) uint16_t smem[WAVES_PER_WG][NV];
+    __shared__ uint32_t hist_s[HIST_LDS];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int i = threadIdx.x; i < HIST_LDS; i += blockDim.x) hist_s[i] = 0u;
+    __syncthreads();
     uint16_t *Ls = smem[wv];
     uint64_t blocks = 0, fe = 0, be = 0, its = 0;
+    WorkSplit ws = work_split(n_cw);
     for (;;) {
-        const uint32_t cw = next_codeword(queue, lane);
+        const uint32_t cw = next_codeword(ws, queue, lane);
         if (cw >= n_cw) break;
         wave_gen_channel(Ls, lane, seed, first_cw + cw, mu, sigma);
         wave_lds_fence();
@@ -297,7 +347,12 @@ __global__ void __launch_bounds__(64 * WAVES_PER_WG)
             fe += 1;
             be += wave_info_errors(Ls, lane);
         }
-        if (lane == 0) atomicAdd(hist + it, 1ull);
+        if (lane == 0) {
+            if (it < HIST_LDS)
+                atomicAdd(hist_s + it, 1u);
+            else
+                atomicAdd(hist + it, 1ull);
+        }
         wave_lds_fence();
     }
     if (lane == 0 && blocks) {
@@ -306,6 +361,10 @@ __global__ void __launch_bounds__(64 * WAVES_PER_WG)
         if (be) atomicAdd(counters + 2, (unsigned long long)be);
         atomicAdd(counters + 3, (unsigned long long)its);
     }
+    __syncthreads();
+    const int nb = max_iter + 1 < HIST_LDS ? max_iter + 1 : HIST_LDS;
+    for (int i = threadIdx.x; i < nb; i += blockDim.x)
+        if (hist_s[i]) atomicAdd(hist + i, (unsigned long long)hist_s[i]);
 }
 
 __global__ void __launch_bounds__(256)

@@ -44,29 +44,51 @@ typedef enum ldpc_status {
 enum {
     /* Tree Approximate-Min* with bf16 rounding of q, r and L, truncated
      * constants and damping beta: the reference's golden stepper
-     * (tests/test_circulant_dma_stepper.cpp:107-327, :743-815). */
-    LDPC_PROFILE_T = 0
+     * (tests/test_circulant_dma_stepper.cpp:107-327, :743-815).
+     * AR4JA r1/2 K=1024 only. */
+    LDPC_PROFILE_T = 0,
+    /* Recursive Approximate-Min* of the scalar_5w engine that
+     * run_simulation.py drives (engines/scalar_5w/ldpc_decoder_core.h:
+     * 155-201, 413-520): unrounded q, rows in the given order, syndrome on
+     * the sign bit from the third iteration, frame error <=> a transmitted
+     * sign bit set.  Check degrees 1..12. */
+    LDPC_PROFILE_R = 1,
+    /* Same engine built with USE_NORMALIZED_MIN_SUM: min-sum, alpha 0.75
+     * (ldpc_decoder_core.h:320-411; the engine's -n flag). */
+    LDPC_PROFILE_R_NMS = 2
 };
 
 /* Code families. */
 enum {
-    LDPC_CODE_AR4JA = 1 /* CCSDS AR4JA (matrices/makeH_AR4JA.m) */
+    LDPC_CODE_AR4JA = 1, /* CCSDS AR4JA (matrices/makeH_AR4JA.m) */
+    LDPC_CODE_MATRIX = 2 /* explicit parity-check matrix (a parsed .chinn.out) */
 };
 
 typedef struct ldpc_code_desc {
-    uint32_t family;   /* LDPC_CODE_AR4JA */
-    uint32_t k_info;   /* 1024, 4096 or 16384 */
-    uint32_t rate_num; /* 1, 2 or 4 */
-    uint32_t rate_den; /* 2, 3 or 5 */
+    uint32_t family;   /* LDPC_CODE_AR4JA or LDPC_CODE_MATRIX */
+    uint32_t k_info;   /* AR4JA: 1024, 4096 or 16384 */
+    uint32_t rate_num; /* AR4JA: 1, 2 or 4 */
+    uint32_t rate_den; /* AR4JA: 2, 3 or 5 */
+    /* LDPC_CODE_MATRIX: m rows in decoding order, each row's variables in
+     * order (row_ptr[m+1], col[row_ptr[m]], 0-based); the last n_punct
+     * variables are punctured.  Copied at ldpc_create. */
+    uint32_t n, m, n_punct, reserved;
+    const int32_t *row_ptr;
+    const int32_t *col;
 } ldpc_code_desc;
 
 typedef struct ldpc_decoder_cfg {
-    uint32_t profile;  /* LDPC_PROFILE_T */
-    uint32_t max_iter; /* >= 1 (reference default 200) */
-    float beta;        /* damping of check messages; 0.9375 matches the
-                          silicon and the stepper's --load-init default
+    uint32_t profile;  /* LDPC_PROFILE_T, _R or _R_NMS */
+    uint32_t max_iter; /* >= 1 (reference default 200; run_simulation.py 16) */
+    float beta;        /* Profile T damping of check messages; 0.9375 matches
+                          the silicon and the stepper's --load-init default
                           (compute_trisc.cpp:395,566; stepper:1017) */
-    uint32_t reserved[5];
+    /* Profile R clamps, 0 = off (ldpc_sim_scalar.cpp:290-309: -L -R -V -G) */
+    float l_max;       /* |channel LLR| <= l_max before rounding */
+    float r_max;       /* |check message| <= r_max */
+    float l_post_max;  /* |posterior LLR| <= l_post_max */
+    float gamma;       /* if > 0: l_max = gamma * mu at each Eb/N0 */
+    uint32_t reserved;
 } ldpc_decoder_cfg;
 
 typedef struct ldpc_code_info {
@@ -121,6 +143,12 @@ int ldpc_get_code_info(const ldpc_ctx *ctx, ldpc_code_info *info);
  * variable of check j of a layer = base + (shift + j) mod z.  Returns the
  * number of int32 written or a negative status. */
 int ldpc_get_schedule(const ldpc_code_desc *code, int32_t *out, int32_t cap);
+
+/* Level of every row for the Profile R schedule (needs no GPU): level(i) =
+ * 1 + max level of the earlier rows sharing a variable with row i.  Rows of
+ * one level are decoded in parallel; the result equals the sequential row
+ * order.  Writes m levels (1-based) and returns the number of levels. */
+int ldpc_row_levels(uint32_t n, uint32_t m, const int32_t *row_ptr, const int32_t *col, int32_t *level);
 
 /* Monte Carlo point: generate, decode and tally codewords first_cw.. until
  * max_blocks or target_fe is reached, in launches of chunk_cw codewords.

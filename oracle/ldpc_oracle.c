@@ -20,7 +20,11 @@
  *       tests/test_circulant_dma_stepper.cpp:861-990
  *
  * and the "Profile R" recursive decoder that run_simulation.py drives
- * (engines/scalar_5w/ldpc_decoder_core.h:155-201, 214-586).
+ * (engines/scalar_5w/ldpc_decoder_core.h:155-201, 214-586): unrounded q,
+ * min / argmin with the first index winning ties, recursive min* with full
+ * fp32 constants (or normalized min-sum, alpha 0.75), optional message and
+ * posterior clamps, syndrome on the raw sign bit from the third iteration on,
+ * frame error <=> a transmitted sign bit set.
  *
  * The channel generator (Philox4x32-10 counter-based stream, Salmon et al.,
  * SC'11, then Box-Muller 1958 with polynomial log/sin/cos and a Newton
@@ -217,6 +221,154 @@ EXPORT int orc_trace_t(int n, int m, const int32_t *rp, const int32_t *col, cons
 }
 
 /* ------------------------------------------------------------------------- */
+/* Profile R: recursive Approx-Min* (engines/scalar_5w/ldpc_decoder_core.h)  */
+/* ------------------------------------------------------------------------- */
+/* Correction g(diff) with full fp32 constants (ldpc_decoder_core.h:162,192). */
+static inline float corr_r(float diff) {
+    return (diff >= 2.0f) ? 0.0f : ((diff < 0.5f) ? (0.69315f - 0.40f * diff) : (0.49315f - 0.328f * (diff - 0.5f)));
+}
+static inline float absbits(float x) { return u2f(f2u(x) & 0x7FFFFFFFu); }
+
+/* ldpc_decoder_core.h:155-173 */
+static void amin_r_deg3(const float *q, uint32_t min_idx, float min_mag, float *r_min, float *r_all) {
+    float o0 = (min_idx == 0) ? absbits(q[1]) : absbits(q[0]);
+    float o1 = (min_idx == 2) ? absbits(q[1]) : absbits(q[2]);
+    float diff = (o0 > o1) ? (o0 - o1) : (o1 - o0);
+    float m_min = (o0 < o1) ? o0 : o1;
+    float corr = corr_r(diff);
+    float m_oth = (m_min > corr) ? (m_min - corr) : 0.0f;
+    float diff_all = (m_oth > min_mag) ? (m_oth - min_mag) : (min_mag - m_oth);
+    float m_all_min = (m_oth < min_mag) ? m_oth : min_mag;
+    float corr_all = corr_r(diff_all);
+    *r_min = m_oth;
+    *r_all = (m_all_min > corr_all) ? (m_all_min - corr_all) : 0.0f;
+}
+
+/* ldpc_decoder_core.h:176-201 */
+static void amin_r_general(const float *q, uint32_t deg, uint32_t min_idx, float min_mag, float *r_min, float *r_all) {
+    uint32_t first = (min_idx == 0) ? 1u : 0u;
+    float m = (deg > 1) ? absbits(q[first]) : min_mag;
+    for (uint32_t d = first + 1; d < deg; ++d) {
+        if (d == min_idx) continue;
+        float a = absbits(q[d]);
+        float diff = (m > a) ? (m - a) : (a - m);
+        float cur = (m < a) ? m : a;
+        float nx = cur - corr_r(diff);
+        m = (nx > 0.0f) ? nx : 0.0f;
+    }
+    float diff_all = (m > min_mag) ? (m - min_mag) : (min_mag - m);
+    float cur = (m < min_mag) ? m : min_mag;
+    float nx = cur - corr_r(diff_all);
+    *r_min = m;
+    *r_all = (nx > 0.0f) ? nx : 0.0f;
+}
+
+typedef struct {
+    int nms;            /* 0: Approx-Min*; 1: normalized min-sum, alpha 0.75 */
+    float r_max;        /* check message clamp, 0 = off */
+    float l_post_max;   /* posterior clamp, 0 = off */
+} orc_cfg_r;
+
+/* One Profile R row update on one codeword (ldpc_decoder_core.h:413-520,
+ * NMS :320-411).  v[] in file order, R[] the row's messages. */
+static void row_update_r(uint16_t *L, uint16_t *R, const int32_t *v, uint32_t deg, const orc_cfg_r *cfg) {
+    float q[32];
+    uint32_t sign = 0, idx = 0;
+    float r_min, r_all;
+    if (cfg->nms) {
+        float min1 = 999.0f, min2 = 999.0f;
+        for (uint32_t d = 0; d < deg; ++d) {
+            q[d] = bf2f(L[v[d]]) - bf2f(R[d]);
+            sign ^= f2u(q[d]) >> 31;
+            float a = absbits(q[d]);
+            if (a < min1) { min2 = min1; min1 = a; idx = d; }
+            else if (a < min2) { min2 = a; }
+        }
+        r_min = 0.75f * min2;
+        r_all = 0.75f * min1;
+    } else {
+        float min_mag = 9999.0f;
+        for (uint32_t d = 0; d < deg; ++d) {
+            q[d] = bf2f(L[v[d]]) - bf2f(R[d]);
+            sign ^= f2u(q[d]) >> 31;
+            float a = absbits(q[d]);
+            if (a < min_mag) { min_mag = a; idx = d; }
+        }
+        if (deg == 3) amin_r_deg3(q, idx, min_mag, &r_min, &r_all);
+        else amin_r_general(q, deg, idx, min_mag, &r_min, &r_all);
+    }
+    for (uint32_t d = 0; d < deg; ++d) {
+        float mag = (d == idx) ? r_min : r_all;
+        if (cfg->r_max > 0.0f && mag > cfg->r_max) mag = cfg->r_max;
+        uint32_t msg_sign = sign ^ (f2u(q[d]) >> 31);
+        float rn = u2f(f2u(mag) | (msg_sign << 31));
+        float ln = q[d] + rn;
+        if (cfg->l_post_max > 0.0f) {
+            if (ln > cfg->l_post_max) ln = cfg->l_post_max;
+            else if (ln < -cfg->l_post_max) ln = -cfg->l_post_max;
+        }
+        L[v[d]] = f2bf(ln);
+        R[d] = f2bf(rn);
+    }
+}
+
+/* Syndrome with hard bit = bf16 sign bit (ldpc_decoder_core.h:526-553). */
+static int syndrome_ok_r(const uint16_t *L, int m, const int32_t *rp, const int32_t *col) {
+    for (int i = 0; i < m; ++i) {
+        uint32_t p = 0;
+        for (int e = rp[i]; e < rp[i + 1]; ++e) p ^= L[col[e]] >> 15;
+        if (p & 1u) return 0;
+    }
+    return 1;
+}
+
+/* Decode one codeword; rows in index order.  Returns the iteration at which
+ * the syndrome cleared (checked from the third iteration on), or 0. */
+static int decode_one_r(uint16_t *L, uint16_t *R, int m, const int32_t *rp, const int32_t *col, int max_iter,
+                        const orc_cfg_r *cfg) {
+    for (int e = 0; e < rp[m]; ++e) R[e] = 0;
+    for (int it = 0; it < max_iter; ++it) {
+        for (int i = 0; i < m; ++i) row_update_r(L, R + rp[i], col + rp[i], (uint32_t)(rp[i + 1] - rp[i]), cfg);
+        if (it >= 2 && syndrome_ok_r(L, m, rp, col)) return it + 1;
+    }
+    return 0;
+}
+
+static int check_degrees_r(int m, const int32_t *rp) {
+    for (int i = 0; i < m; ++i) {
+        int d = rp[i + 1] - rp[i];
+        if (d < 1 || d > 12) return -1;   /* ldpc_sim_scalar.cpp:34,187 */
+    }
+    return 0;
+}
+
+EXPORT int orc_decode_r(int n, int m, const int32_t *rp, const int32_t *col, int n_cw, const uint16_t *llr_in,
+                        int max_iter, int nms, float r_max, float l_post_max, uint16_t *llr_out, uint32_t *out_iters,
+                        uint8_t *out_ok, int threads) {
+    if (check_degrees_r(m, rp)) return -1;
+    orc_cfg_r cfg = {nms, r_max, l_post_max};
+#ifdef _OPENMP
+    if (threads <= 0) threads = omp_get_max_threads();
+#pragma omp parallel num_threads(threads)
+#endif
+    {
+        uint16_t *R = (uint16_t *)malloc(sizeof(uint16_t) * (size_t)rp[m]);
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 4)
+#endif
+        for (int c = 0; c < n_cw; ++c) {
+            uint16_t *L = llr_out + (size_t)c * n;
+            memcpy(L, llr_in + (size_t)c * n, sizeof(uint16_t) * n);
+            int it = decode_one_r(L, R, m, rp, col, max_iter, &cfg);
+            out_iters[c] = it ? (uint32_t)it : (uint32_t)max_iter;
+            out_ok[c] = it ? 1 : 0;
+        }
+        free(R);
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------------- */
 /* Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11)                          */
 /* ------------------------------------------------------------------------- */
 EXPORT void orc_philox4x32_10(const uint32_t ctr_in[4], const uint32_t key_in[2], uint32_t out[4]) {
@@ -315,7 +467,16 @@ EXPORT void orc_box_muller(uint32_t x0, uint32_t x1, float *z0, float *z1) {
  * key = (seed_lo, seed_hi)) word pair (i%4)&2 and normal i%4; the LLR is
  * RNE_bf16(mu + sigma * z).  Variables n_tx..n-1 are punctured and set to +0
  * (ldpc_decoder_core.h:295-297; stepper:748-749).                           */
+static void gen_one_clamped(const uint32_t key[2], uint64_t w, float mu, float sigma, float l_max, int n_tx, int n,
+                            uint16_t *L);
 static void gen_one(const uint32_t key[2], uint64_t w, float mu, float sigma, int n_tx, int n, uint16_t *L) {
+    gen_one_clamped(key, w, mu, sigma, 0.0f, n_tx, n, L);
+}
+
+/* Channel LLRs with the optional magnitude clamp l_max applied before
+ * rounding (ldpc_decoder_core.h:266-275). */
+static void gen_one_clamped(const uint32_t key[2], uint64_t w, float mu, float sigma, float l_max, int n_tx, int n,
+                            uint16_t *L) {
     for (int blk = 0; blk < (n_tx + 3) / 4; ++blk) {
         uint32_t ctr[4] = {(uint32_t)w, (uint32_t)(w >> 32), (uint32_t)blk, 0u}, x[4];
         orc_philox4x32_10(ctr, key, x);
@@ -324,7 +485,12 @@ static void gen_one(const uint32_t key[2], uint64_t w, float mu, float sigma, in
         orc_box_muller(x[2], x[3], &z[2], &z[3]);
         for (int j = 0; j < 4 && 4 * blk + j < n_tx; ++j) {
             float t = sigma * z[j];
-            L[4 * blk + j] = f2bf(mu + t);
+            float l = mu + t;
+            if (l_max > 0.0f) {
+                if (l > l_max) l = l_max;
+                else if (l < -l_max) l = -l_max;
+            }
+            L[4 * blk + j] = f2bf(l);
         }
     }
     for (int v = n_tx; v < n; ++v) L[v] = 0;
@@ -407,6 +573,60 @@ EXPORT int orc_run_point_t(int n, int m, const int32_t *rp, const int32_t *col, 
     }
     counters[0] = blocks; counters[1] = fe; counters[2] = be; counters[3] = its;
     return 0;
+}
+
+/* Monte Carlo point, Profile R (ldpc_decoder_core.h:258-577): frame error
+ * <=> at least one sign bit set among the n_tx transmitted variables; bit
+ * errors count those sign bits.  hist / iteration sum as in Profile T, with
+ * "converged" meaning the syndrome cleared (checked from iteration 3). */
+EXPORT int orc_run_point_r(int n, int m, const int32_t *rp, const int32_t *col, int n_tx, uint64_t seed,
+                           uint64_t first_cw, uint64_t n_cw, float mu, float sigma, float l_max, int max_iter, int nms,
+                           float r_max, float l_post_max, int threads, uint64_t *counters, uint64_t *hist) {
+    if (check_degrees_r(m, rp)) return -1;
+    orc_cfg_r cfg = {nms, r_max, l_post_max};
+    uint64_t blocks = 0, fe = 0, be = 0, its = 0;
+    const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    memset(hist, 0, sizeof(uint64_t) * (size_t)(max_iter + 1));
+#ifdef _OPENMP
+    if (threads <= 0) threads = omp_get_max_threads();
+#pragma omp parallel num_threads(threads) reduction(+ : blocks, fe, be, its)
+#endif
+    {
+        uint16_t *L = (uint16_t *)malloc(sizeof(uint16_t) * (size_t)n);
+        uint16_t *R = (uint16_t *)malloc(sizeof(uint16_t) * (size_t)rp[m]);
+        uint64_t *h = (uint64_t *)calloc((size_t)max_iter + 1, sizeof(uint64_t));
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 16)
+#endif
+        for (int64_t c = 0; c < (int64_t)n_cw; ++c) {
+            gen_one_clamped(key, first_cw + (uint64_t)c, mu, sigma, l_max, n_tx, n, L);
+            int it = decode_one_r(L, R, m, rp, col, max_iter, &cfg);
+            blocks += 1;
+            its += it ? (uint64_t)it : (uint64_t)max_iter;
+            h[it] += 1;
+            uint64_t b = 0;
+            for (int k = 0; k < n_tx; ++k) b += L[k] >> 15;
+            be += b;
+            fe += b > 0;
+        }
+#ifdef _OPENMP
+#pragma omp critical
+#endif
+        for (int i = 0; i <= max_iter; ++i) hist[i] += h[i];
+        free(L); free(R); free(h);
+    }
+    counters[0] = blocks; counters[1] = fe; counters[2] = be; counters[3] = its;
+    return 0;
+}
+
+EXPORT void orc_gen_channel_clamped(uint64_t seed, uint64_t first_cw, int n_cw, float mu, float sigma, float l_max,
+                                    int n_tx, int n, uint16_t *out) {
+    const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
+    for (int c = 0; c < n_cw; ++c)
+        gen_one_clamped(key, first_cw + (uint64_t)c, mu, sigma, l_max, n_tx, n, out + (size_t)c * n);
 }
 
 EXPORT int orc_num_threads(void) {

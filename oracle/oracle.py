@@ -64,6 +64,16 @@ def lib():
         L.orc_run_point_t.argtypes = [C.c_int, C.c_int, i32p, i32p, i32p, C.c_int, C.c_int, C.c_uint64,
                                       C.c_uint64, C.c_uint64, C.c_float, C.c_float, C.c_int, C.c_float, C.c_int,
                                       u64p, u64p]
+        L.orc_decode_r.restype = C.c_int
+        L.orc_decode_r.argtypes = [C.c_int, C.c_int, i32p, i32p, C.c_int, u16p, C.c_int, C.c_int, C.c_float, C.c_float,
+                                   u16p, u32p, u8p, C.c_int]
+        L.orc_run_point_r.restype = C.c_int
+        L.orc_run_point_r.argtypes = [C.c_int, C.c_int, i32p, i32p, C.c_int, C.c_uint64, C.c_uint64, C.c_uint64,
+                                      C.c_float, C.c_float, C.c_float, C.c_int, C.c_int, C.c_float, C.c_float, C.c_int,
+                                      u64p, u64p]
+        L.orc_gen_channel_clamped.restype = None
+        L.orc_gen_channel_clamped.argtypes = [C.c_uint64, C.c_uint64, C.c_int, C.c_float, C.c_float, C.c_float, C.c_int,
+                                              C.c_int, u16p]
         L.orc_num_threads.restype = C.c_int
         L.orc_num_threads.argtypes = []
         _lib = L
@@ -155,6 +165,47 @@ def run_point_t(code, seed: int, first_cw: int, n_cw: int, mu: float, sigma: flo
         raise ValueError("oracle Profile T supports check degrees 3 and 6 only")
     return dict(blocks=int(cnt[0]), frame_errors=int(cnt[1]), bit_errors=int(cnt[2]), iter_sum=int(cnt[3]),
                 hist=hist)
+
+
+def rows_arrays(rows):
+    """(row_ptr, col) int32 arrays of rows given in decoding (file) order."""
+    rp = np.zeros(len(rows) + 1, dtype=np.int32)
+    rp[1:] = np.cumsum([len(r) for r in rows])
+    col = np.concatenate([np.asarray(r, dtype=np.int32) for r in rows]).astype(np.int32)
+    return rp, col
+
+
+def decode_r(n, rows, llr, max_iter=16, nms=False, r_max=0.0, l_post_max=0.0, threads=0):
+    """Profile R decode of bf16 LLRs [n_cw, n]; rows in file order."""
+    llr = np.ascontiguousarray(llr, dtype=np.uint16).reshape(-1, n)
+    rp, col = rows_arrays(rows)
+    out = np.empty_like(llr)
+    iters = np.empty(llr.shape[0], dtype=np.uint32)
+    ok = np.empty(llr.shape[0], dtype=np.uint8)
+    rc = lib().orc_decode_r(n, len(rows), rp, col, llr.shape[0], llr, max_iter, int(nms), r_max, l_post_max, out,
+                            iters, ok, threads)
+    if rc:
+        raise ValueError("Profile R supports check degrees 1..12")
+    return out, iters, ok
+
+
+def run_point_r(n, rows, n_tx, seed, first_cw, n_cw, mu, sigma, l_max=0.0, max_iter=16, nms=False, r_max=0.0,
+                l_post_max=0.0, threads=0):
+    rp, col = rows_arrays(rows)
+    cnt = np.zeros(4, dtype=np.uint64)
+    hist = np.zeros(max_iter + 1, dtype=np.uint64)
+    rc = lib().orc_run_point_r(n, len(rows), rp, col, n_tx, seed, first_cw, n_cw, mu, sigma, l_max, max_iter,
+                               int(nms), r_max, l_post_max, threads, cnt, hist)
+    if rc:
+        raise ValueError("Profile R supports check degrees 1..12")
+    return dict(blocks=int(cnt[0]), frame_errors=int(cnt[1]), bit_errors=int(cnt[2]), iter_sum=int(cnt[3]),
+                hist=hist)
+
+
+def gen_channel_clamped(n, n_tx, seed, first_cw, n_cw, mu, sigma, l_max=0.0):
+    out = np.empty((n_cw, n), dtype=np.uint16)
+    lib().orc_gen_channel_clamped(seed, first_cw, n_cw, mu, sigma, l_max, n_tx, n, out)
+    return out
 
 
 def num_threads() -> int:

@@ -35,7 +35,7 @@ def test_every_declared_symbol_is_exported():
 
 def test_struct_layouts_match_header():
     # fixed-width POD structs: sizes a C compiler gives them on x86-64
-    assert C.sizeof(L.CodeDesc) == 16
+    assert C.sizeof(L.CodeDesc) == 48
     assert C.sizeof(L.DecoderCfg) == 32
     assert C.sizeof(L.CodeInfo) == 32
     assert C.sizeof(L.PointCfg) == 48
@@ -78,6 +78,30 @@ def test_schedule_of_unbuilt_code_is_unsupported():
     buf = (C.c_int32 * 64)()
     desc = L.CodeDesc(L.LDPC_CODE_AR4JA, 4096, 2, 3)
     assert lib.ldpc_get_schedule(C.byref(desc), buf, 64) == L.LDPC_ERR_UNSUPPORTED
+
+
+def test_row_levels_rejects_bad_matrices():
+    lib = L.load()
+    rp = (C.c_int32 * 3)(0, 2, 15)
+    col = (C.c_int32 * 15)(*range(15))
+    out = (C.c_int32 * 2)()
+    assert lib.ldpc_row_levels(20, 2, rp, col, out) == L.LDPC_ERR_ARG   # degree 13 > 12
+    rp2 = (C.c_int32 * 3)(0, 2, 4)
+    assert lib.ldpc_row_levels(3, 2, rp2, col, out) == L.LDPC_ERR_ARG   # variable 3 out of range
+    assert lib.ldpc_row_levels(4, 2, rp2, col, out) == 1 and list(out) == [1, 1]
+
+
+def test_profile_t_rejects_clamps_and_matrices():
+    lib = L.load()
+    h = C.c_void_p()
+    code = L.CodeDesc(L.LDPC_CODE_AR4JA, 1024, 1, 2)
+    cfg = L.DecoderCfg(L.LDPC_PROFILE_T, 200, 0.9375, 8.0)
+    assert lib.ldpc_create(C.byref(code), C.byref(cfg), 0, C.byref(h)) == L.LDPC_ERR_ARG
+    cfg = L.DecoderCfg(L.LDPC_PROFILE_R, 16, 0.0, -1.0)
+    assert lib.ldpc_create(C.byref(code), C.byref(cfg), 0, C.byref(h)) == L.LDPC_ERR_ARG
+    bad = L.CodeDesc(L.LDPC_CODE_MATRIX, 0, 0, 0, 10, 0, 0, 0, None, None)
+    cfg = L.DecoderCfg(L.LDPC_PROFILE_R, 16, 0.0)
+    assert lib.ldpc_create(C.byref(bad), C.byref(cfg), 0, C.byref(h)) == L.LDPC_ERR_ARG
 
 
 def test_missing_library_fails_loudly(monkeypatch, tmp_path):

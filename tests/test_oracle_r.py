@@ -1,0 +1,102 @@
+"""Profile R oracle (scalar_5w recursive Approx-Min* / normalized min-sum),
+pinned against the reference's own worker (tests/golden/scalar_profile_r.npz,
+written by oracle/_ref/scalar_driver around ldpc_decoder_core.h)."""
+import hashlib
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN_DIR, REFERENCE
+from ldpc_mi355x import codes
+
+MATRIX = os.path.join(GOLDEN_DIR, "AR4JA_r45_4c_128c_r12.chinn.out")
+
+
+@pytest.fixture(scope="module")
+def scalar_golden():
+    return np.load(os.path.join(GOLDEN_DIR, "scalar_profile_r.npz"))
+
+
+@pytest.fixture(scope="module")
+def chinn():
+    return codes.read_chinn_out(MATRIX)
+
+
+def test_matrix_copy_is_the_reference_file(scalar_golden):
+    assert hashlib.sha256(open(MATRIX, "rb").read()).hexdigest() == str(scalar_golden["matrix_sha256"])
+    if os.path.isdir(REFERENCE):
+        ref = os.path.join(REFERENCE, "matrices", "AR4JA_r45_4c_128c_r12.chinn.out")
+        assert open(ref, "rb").read() == open(MATRIX, "rb").read()
+
+
+def test_chinn_is_an_ar4ja_r12_labelling(chinn, code12):
+    """Same degree profile as the CCSDS construction (a relabelled H)."""
+    M, N, rows = chinn
+    H = np.zeros((M, N), dtype=np.uint8)
+    for i, r in enumerate(rows):
+        H[i, r] = 1
+    assert np.array_equal(np.sort(H.sum(1)), np.sort(code12.row_degrees()))
+    assert np.array_equal(np.bincount(H[:, 2048:].sum(0)), np.bincount(code12.col_degrees()[2048:]))
+    assert all(r == sorted(r) for r in rows)
+
+
+def test_oracle_matches_reference_worker(scalar_golden, chinn, oracle):
+    M, N, rows = chinn
+    z = scalar_golden
+    for i in range(len(z["nms"])):
+        out, iters, ok = oracle.decode_r(N, rows, z["llr_in"][i], int(z["max_iter"][i]), bool(z["nms"][i]),
+                                         float(z["r_max"][i]), float(z["l_post_max"][i]))
+        np.testing.assert_array_equal(out, z["llr_out"][i], err_msg=f"case {i}")
+        be = [int((out[c, :2048] >> 15).sum()) for c in range(2)]
+        assert sum(be) == z["stats"][i][0] and sum(b > 0 for b in be) == z["stats"][i][1]
+        assert int(iters[0]) == z["stats"][i][2]
+
+
+def test_fixture_covers_variants(scalar_golden):
+    z = scalar_golden
+    assert set(z["nms"]) == {0, 1}
+    assert (z["l_max"] > 0).any() and (z["r_max"] > 0).any() and (z["l_post_max"] > 0).any()
+    assert (z["max_iter"] <= 3).any() and (z["stats"][:, 1] > 0).any() and (z["stats"][:, 1] == 0).any()
+
+
+def test_syndrome_starts_at_third_iteration(chinn, oracle):
+    """A clean channel converges in one iteration but is only detected at 3."""
+    M, N, rows = chinn
+    llr = np.full((1, N), 0x4100, dtype=np.uint16)  # +8.0
+    llr[0, 2048:] = 0
+    out, iters, ok = oracle.decode_r(N, rows, llr, 10)
+    assert ok[0] == 1 and iters[0] == 3
+
+
+def test_r_levels_are_sequentially_equivalent(chinn):
+    """ASAP levels of the file row order (37 for the shipped matrix): for every
+    variable, the rows touching it have strictly increasing levels in file
+    order, so level-parallel decoding reads what the sequential order reads;
+    libldpc's C++ levels equal the host ones."""
+    from ldpc_mi355x import decoder, schedule_r
+    M, N, rows = chinn
+    lv = schedule_r.row_levels(rows, N)
+    assert max(lv) == 37
+    last = {}
+    for i, r in enumerate(rows):
+        for v in r:
+            assert last.get(v, 0) < lv[i]
+            last[v] = lv[i]
+    assert np.array_equal(decoder.row_levels(rows, N), lv)
+    assert sum(len(x) for x in schedule_r.levels(rows, N)) == M
+
+
+def test_level_parallel_decode_equals_sequential(chinn, oracle):
+    """Re-ordering rows level by level (what the GPU does) leaves the oracle's
+    sequential Profile R result unchanged."""
+    from ldpc_mi355x import schedule_r
+    from conftest import channel_params
+    M, N, rows = chinn
+    mu, sg = channel_params(1.2)
+    llr = oracle.gen_channel_clamped(N, 2048, 21, 0, 64, mu, sg)
+    seq, it1, _ = oracle.decode_r(N, rows, llr, 40)
+    order = [i for lv in schedule_r.levels(rows, N) for i in lv]
+    par, it2, _ = oracle.decode_r(N, [rows[i] for i in order], llr, 40)
+    np.testing.assert_array_equal(seq, par)
+    np.testing.assert_array_equal(it1, it2)

@@ -1,7 +1,9 @@
 // runtime.cpp -- the C ABI of libldpc (include/ldpc_mi355x.h): contexts,
-// device buffers, the chunked Monte Carlo driver and its stopping rule.
+// device buffers, code set-up for both decoder profiles, the chunked Monte
+// Carlo driver and its stopping rule.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -10,6 +12,7 @@
 #include <vector>
 
 #include "../../include/ldpc_mi355x.h"
+#include "ar4ja_schedule.h"
 #include "kernels.h"
 
 namespace {
@@ -34,6 +37,9 @@ struct ldpc_ctx {
     uint32_t *d_queue = nullptr;
     unsigned long long *d_counters = nullptr;  // [4]
     unsigned long long *d_hist = nullptr;      // [max_iter + 1]
+    // Profile R: level-ordered pass table on the device
+    void *d_table = nullptr;
+    ldpc::RLaunch rl{};
     DevBuf in, out, iters, ok;
     std::string err;
 };
@@ -69,25 +75,168 @@ bool is_r12_k1024(const ldpc_code_desc &d) {
     return d.family == LDPC_CODE_AR4JA && d.k_info == 1024 && d.rate_num == 1 && d.rate_den == 2;
 }
 
+bool ar4ja_ids(const ldpc_code_desc &d, int &k_id, int &rate_id) {
+    k_id = d.k_info == 1024 ? 0 : d.k_info == 4096 ? 1 : d.k_info == 16384 ? 2 : -1;
+    rate_id = (d.rate_num == 1 && d.rate_den == 2)   ? 0
+              : (d.rate_num == 2 && d.rate_den == 3) ? 1
+              : (d.rate_num == 4 && d.rate_den == 5) ? 2
+                                                     : -1;
+    return k_id >= 0 && rate_id >= 0;
+}
+
+// Parity-check rows of an AR4JA code in the natural (makeH) row order, each
+// row's variables ascending.
+void ar4ja_rows(int k_id, int rate_id, std::vector<int32_t> &rp, std::vector<int32_t> &col, uint32_t &n,
+                uint32_t &m, uint32_t &np) {
+    const ar4ja::Schedule s = ar4ja::make_schedule(k_id, rate_id);
+    n = (uint32_t)s.N;
+    m = (uint32_t)s.n_checks;
+    np = (uint32_t)s.n_punct;
+    std::vector<const ar4ja::Layer *> by_first(s.n_checks / s.Z);
+    for (int li = 0; li < s.n_layers; ++li) by_first[s.layer[li].first_row / s.Z] = &s.layer[li];
+    rp.assign(1, 0);
+    col.clear();
+    for (int r = 0; r < s.n_checks; ++r) {
+        const ar4ja::Layer &L = *by_first[r / s.Z];
+        const int j = r % s.Z;
+        for (int d = 0; d < L.deg; ++d) col.push_back(L.slot[d].base + (L.slot[d].shift + j) % s.Z);
+        rp.push_back((int32_t)col.size());
+    }
+}
+
+int levelize(uint32_t n, uint32_t m, const int32_t *rp, const int32_t *col, int32_t *level) {
+    std::vector<int32_t> last(n, 0);
+    int n_levels = 0;
+    for (uint32_t i = 0; i < m; ++i) {
+        int l = 0;
+        for (int32_t e = rp[i]; e < rp[i + 1]; ++e) l = std::max(l, last[col[e]]);
+        level[i] = l + 1;
+        for (int32_t e = rp[i]; e < rp[i + 1]; ++e) last[col[e]] = l + 1;
+        n_levels = std::max(n_levels, l + 1);
+    }
+    return n_levels;
+}
+
+int validate_rows(uint32_t n, uint32_t m, const int32_t *rp, const int32_t *col, std::string &why) {
+    if (!rp || !col || n == 0 || m == 0) {
+        why = "matrix: empty or null arrays";
+        return LDPC_ERR_ARG;
+    }
+    if (rp[0] != 0) {
+        why = "matrix: row_ptr[0] must be 0";
+        return LDPC_ERR_ARG;
+    }
+    for (uint32_t i = 0; i < m; ++i) {
+        const int d = rp[i + 1] - rp[i];
+        if (d < 1 || d > 12) {
+            why = "matrix: check degree must be 1..12 (ldpc_sim_scalar.cpp:34)";
+            return LDPC_ERR_ARG;
+        }
+        for (int32_t e = rp[i]; e < rp[i + 1]; ++e)
+            if (col[e] < 0 || (uint32_t)col[e] >= n) {
+                why = "matrix: variable index out of range";
+                return LDPC_ERR_ARG;
+            }
+    }
+    return LDPC_OK;
+}
+
+// Profile R set-up: levels -> passes of <= 64 rows -> device table.
+int setup_profile_r(ldpc_ctx *c, const std::vector<int32_t> &rp, const std::vector<int32_t> &col, uint32_t n,
+                    uint32_t m, uint32_t np, uint32_t k_info) {
+    std::string why;
+    int rc = validate_rows(n, m, rp.data(), col.data(), why);
+    if (rc) return fail(nullptr, rc, why);
+    if (n > 65535) return fail(nullptr, LDPC_ERR_UNSUPPORTED, "Profile R: more than 65535 variables");
+    const uint32_t n_edges = (uint32_t)rp[m];
+    if (n_edges >= (1u << 28) || ldpc::r_lds_bytes((int)n, (int)n_edges) > 65536)
+        return fail(nullptr, LDPC_ERR_UNSUPPORTED,
+                    "Profile R keeps a codeword's LLRs and messages in LDS: needs n + edges <= 16384");
+    std::vector<int32_t> level(m);
+    const int n_levels = levelize(n, m, rp.data(), col.data(), level.data());
+    int maxd = 0;
+    for (uint32_t i = 0; i < m; ++i) maxd = std::max(maxd, rp[i + 1] - rp[i]);
+    const int words = maxd <= 6 ? 4 : 8;  // uint32 per lane entry
+    std::vector<std::vector<uint32_t>> by_level(n_levels + 1);
+    for (uint32_t i = 0; i < m; ++i) by_level[level[i]].push_back(i);
+    std::vector<uint32_t> table;
+    int n_pass = 0;
+    for (int l = 1; l <= n_levels; ++l) {
+        const auto &rows = by_level[l];
+        for (size_t off = 0; off < rows.size(); off += 64) {
+            const size_t base = table.size();
+            table.resize(base + 64 * (size_t)words, 0u);
+            for (size_t k = 0; k < 64 && off + k < rows.size(); ++k) {
+                const uint32_t r = rows[off + k];
+                uint32_t *e = &table[base + k * words];
+                const uint32_t deg = (uint32_t)(rp[r + 1] - rp[r]);
+                e[0] = deg | ((uint32_t)rp[r] << 4);
+                // variables 0..5 in words 1..3, 6..11 in words 4..6
+                for (uint32_t d = 0; d < deg; ++d)
+                    e[1 + d / 2] |= ((uint32_t)col[rp[r] + d] & 0xFFFFu) << (16 * (d & 1));
+            }
+            ++n_pass;
+        }
+    }
+    HIP_TRY(c, hipMalloc(&c->d_table, table.size() * sizeof(uint32_t)));
+    HIP_TRY(c, hipMemcpy(c->d_table, table.data(), table.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    c->rl.table = c->d_table;
+    c->rl.n_pass = n_pass;
+    c->rl.n = (int)n;
+    c->rl.n_tx = (int)(n - np);
+    c->rl.n_edges = (int)n_edges;
+    c->rl.max_iter = (int)c->cfg.max_iter;
+    c->rl.r_max = c->cfg.r_max;
+    c->rl.l_post_max = c->cfg.l_post_max;
+    c->rl.l_max = c->cfg.l_max;
+    c->rl.maxd = maxd;
+    c->rl.nms = c->cfg.profile == LDPC_PROFILE_R_NMS;
+    c->info.n = n;
+    c->info.m = m;
+    c->info.k_info = k_info;
+    c->info.n_tx = n - np;
+    c->info.z = 0;
+    c->info.n_layers = (uint32_t)n_levels;
+    c->info.n_edges = n_edges;
+    return LDPC_OK;
+}
+
 }  // namespace
 
 extern "C" {
 
 const char *ldpc_last_error(const ldpc_ctx *ctx) { return ctx ? ctx->err.c_str() : g_create_error.c_str(); }
 
+int ldpc_row_levels(uint32_t n, uint32_t m, const int32_t *row_ptr, const int32_t *col, int32_t *level) {
+    std::string why;
+    if (!level || validate_rows(n, m, row_ptr, col, why)) return LDPC_ERR_ARG;
+    return levelize(n, m, row_ptr, col, level);
+}
+
 int ldpc_create(const ldpc_code_desc *code, const ldpc_decoder_cfg *cfg, int device, ldpc_ctx **out) {
     if (!code || !cfg || !out) return fail(nullptr, LDPC_ERR_ARG, "null argument");
     *out = nullptr;
-    if (code->family != LDPC_CODE_AR4JA) return fail(nullptr, LDPC_ERR_ARG, "unknown code family");
-    const bool known_k = code->k_info == 1024 || code->k_info == 4096 || code->k_info == 16384;
-    const bool known_r = (code->rate_num == 1 && code->rate_den == 2) || (code->rate_num == 2 && code->rate_den == 3) ||
-                         (code->rate_num == 4 && code->rate_den == 5);
-    if (!known_k || !known_r) return fail(nullptr, LDPC_ERR_ARG, "no AR4JA code with this (k_info, rate)");
-    if (cfg->profile != LDPC_PROFILE_T) return fail(nullptr, LDPC_ERR_UNSUPPORTED, "unknown decoder profile");
+    if (code->family != LDPC_CODE_AR4JA && code->family != LDPC_CODE_MATRIX)
+        return fail(nullptr, LDPC_ERR_ARG, "unknown code family");
+    int k_id = -1, rate_id = -1;
+    if (code->family == LDPC_CODE_AR4JA && !ar4ja_ids(*code, k_id, rate_id))
+        return fail(nullptr, LDPC_ERR_ARG, "no AR4JA code with this (k_info, rate)");
+    if (cfg->profile > LDPC_PROFILE_R_NMS) return fail(nullptr, LDPC_ERR_UNSUPPORTED, "unknown decoder profile");
     if (cfg->max_iter < 1 || cfg->max_iter > 100000) return fail(nullptr, LDPC_ERR_ARG, "max_iter out of range");
-    if (!(cfg->beta > 0.0f) || !std::isfinite(cfg->beta)) return fail(nullptr, LDPC_ERR_ARG, "beta must be > 0");
-    if (!is_r12_k1024(*code))
-        return fail(nullptr, LDPC_ERR_UNSUPPORTED, "Profile T kernels are built for AR4JA r1/2 K=1024 only");
+    const bool prof_t = cfg->profile == LDPC_PROFILE_T;
+    if (prof_t) {
+        if (!(cfg->beta > 0.0f) || !std::isfinite(cfg->beta)) return fail(nullptr, LDPC_ERR_ARG, "beta must be > 0");
+        if (cfg->l_max != 0.0f || cfg->r_max != 0.0f || cfg->l_post_max != 0.0f || cfg->gamma != 0.0f)
+            return fail(nullptr, LDPC_ERR_ARG, "clamps belong to Profile R; the golden Profile T has none");
+        if (!is_r12_k1024(*code))
+            return fail(nullptr, LDPC_ERR_UNSUPPORTED, "Profile T kernels are built for AR4JA r1/2 K=1024 only");
+    } else {
+        for (float v : {cfg->l_max, cfg->r_max, cfg->l_post_max, cfg->gamma})
+            if (!(v >= 0.0f) || !std::isfinite(v)) return fail(nullptr, LDPC_ERR_ARG, "clamps must be >= 0");
+        if (code->family == LDPC_CODE_MATRIX &&
+            (!code->row_ptr || !code->col || code->m == 0 || code->n == 0 || code->n_punct >= code->n))
+            return fail(nullptr, LDPC_ERR_ARG, "matrix: bad dimensions or null arrays");
+    }
 
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(nullptr, LDPC_ERR_NODEV, "no HIP device");
@@ -96,14 +245,9 @@ int ldpc_create(const ldpc_code_desc *code, const ldpc_decoder_cfg *cfg, int dev
     ldpc_ctx *c = new ldpc_ctx();
     c->device = device;
     c->code = *code;
+    c->code.row_ptr = nullptr;  // not kept: the matrix is copied into the pass table
+    c->code.col = nullptr;
     c->cfg = *cfg;
-    c->info.n = 2560;
-    c->info.m = 1536;
-    c->info.k_info = 1024;
-    c->info.n_tx = 2048;
-    c->info.z = 128;
-    c->info.n_layers = 12;
-    c->info.n_edges = 7680;
     c->info.max_iter = cfg->max_iter;
 
     auto bail = [&](hipError_t e, const char *what) {
@@ -120,13 +264,56 @@ int ldpc_create(const ldpc_code_desc *code, const ldpc_decoder_cfg *cfg, int dev
     if ((e = hipMalloc(&c->d_counters, 4 * sizeof(unsigned long long))) != hipSuccess) return bail(e, "hipMalloc");
     if ((e = hipMalloc(&c->d_hist, (cfg->max_iter + 1) * sizeof(unsigned long long))) != hipSuccess)
         return bail(e, "hipMalloc");
-    int cus = 0, occ_mc = 0, occ_dec = 0;
+    int cus = 0;
     if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device)) != hipSuccess)
         return bail(e, "hipDeviceGetAttribute");
-    if ((e = ldpc::t12_occupancy(&occ_mc)) != hipSuccess) return bail(e, "occupancy");
-    if ((e = ldpc::t12_occupancy_decode(&occ_dec)) != hipSuccess) return bail(e, "occupancy");
-    c->grid_mc = cus * (occ_mc > 0 ? occ_mc : 1);
-    c->grid_dec = cus * (occ_dec > 0 ? occ_dec : 1);
+
+    if (prof_t) {
+        c->info.n = 2560;
+        c->info.m = 1536;
+        c->info.k_info = 1024;
+        c->info.n_tx = 2048;
+        c->info.z = 128;
+        c->info.n_layers = 12;
+        c->info.n_edges = 7680;
+        int occ_mc = 0, occ_dec = 0;
+        if ((e = ldpc::t12_occupancy(&occ_mc)) != hipSuccess) return bail(e, "occupancy");
+        if ((e = ldpc::t12_occupancy_decode(&occ_dec)) != hipSuccess) return bail(e, "occupancy");
+        c->grid_mc = cus * (occ_mc > 0 ? occ_mc : 1);
+        c->grid_dec = cus * (occ_dec > 0 ? occ_dec : 1);
+    } else {
+        std::vector<int32_t> rp, col;
+        uint32_t n = 0, m = 0, np = 0, k_info = 0;
+        if (code->family == LDPC_CODE_AR4JA) {
+            ar4ja_rows(k_id, rate_id, rp, col, n, m, np);
+            k_info = code->k_info;
+        } else {
+            n = code->n;
+            m = code->m;
+            np = code->n_punct;
+            rp.assign(code->row_ptr, code->row_ptr + m + 1);
+            if (rp[m] <= 0 || rp[m] > (1 << 24)) {
+                ldpc_destroy(c);
+                return fail(nullptr, LDPC_ERR_ARG, "matrix: bad row_ptr");
+            }
+            col.assign(code->col, code->col + rp[m]);
+            k_info = n > m ? n - m : 0;  // rate (N - M) / (N - P) (ldpc_sim_scalar.cpp:352)
+        }
+        const int rc = setup_profile_r(c, rp, col, n, m, np, k_info);
+        if (rc) {
+            const std::string msg = c->err.empty() ? g_create_error : c->err;
+            ldpc_destroy(c);
+            g_create_error = msg;
+            return rc;
+        }
+        int occ = 0;
+        if ((e = ldpc::r_occupancy(c->rl.maxd, c->rl.nms, ldpc::r_lds_bytes(c->rl.n, c->rl.n_edges), &occ)) !=
+            hipSuccess)
+            return bail(e, "occupancy");
+        c->grid_mc = c->grid_dec = cus * (occ > 0 ? occ : 1);
+        c->rl.grid = c->grid_mc;
+        c->rl.queue = c->d_queue;
+    }
     *out = c;
     return LDPC_OK;
 }
@@ -138,6 +325,7 @@ void ldpc_destroy(ldpc_ctx *c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (DevBuf *b : {&c->in, &c->out, &c->iters, &c->ok})
         if (b->p) (void)hipFree(b->p);
+    if (c->d_table) (void)hipFree(c->d_table);
     if (c->d_queue) (void)hipFree(c->d_queue);
     if (c->d_counters) (void)hipFree(c->d_counters);
     if (c->d_hist) (void)hipFree(c->d_hist);
@@ -178,6 +366,8 @@ int ldpc_sync(ldpc_ctx *c) {
 
 void *ldpc_stream(ldpc_ctx *c) { return c ? (void *)c->stream : nullptr; }
 
+static float l_max_at(const ldpc_ctx *c, float mu) { return c->cfg.gamma > 0.0f ? c->cfg.gamma * mu : c->cfg.l_max; }
+
 int ldpc_run_point(ldpc_ctx *c, const ldpc_point_cfg *pt, ldpc_point_result *res, uint64_t *hist) {
     if (!c || !pt || !res) return fail(c, LDPC_ERR_ARG, "null argument");
     if (!std::isfinite(pt->ebn0_db)) return fail(c, LDPC_ERR_ARG, "ebn0_db must be finite");
@@ -187,6 +377,9 @@ int ldpc_run_point(ldpc_ctx *c, const ldpc_point_cfg *pt, ldpc_point_result *res
     HIP_TRY(c, hipSetDevice(c->device));
     float mu, sigma;
     ldpc_channel_params(c, pt->ebn0_db, &mu, &sigma);
+    const bool prof_t = c->cfg.profile == LDPC_PROFILE_T;
+    ldpc::RLaunch rl = c->rl;
+    rl.l_max = l_max_at(c, mu);
     const uint32_t nh = c->cfg.max_iter + 1;
     std::vector<unsigned long long> h(nh);
     unsigned long long cnt[4];
@@ -199,8 +392,12 @@ int ldpc_run_point(ldpc_ctx *c, const ldpc_point_cfg *pt, ldpc_point_result *res
         HIP_TRY(c, ldpc::launch_fill_u64(c->d_counters, 4, 0ull, c->stream));
         HIP_TRY(c, ldpc::launch_fill_u64(c->d_hist, nh, 0ull, c->stream));
         HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
-        HIP_TRY(c, ldpc::t12_launch_mc(pt->seed, next, (uint32_t)n, mu, sigma, (int)c->cfg.max_iter, c->cfg.beta,
-                                       c->d_queue, c->d_counters, c->d_hist, c->grid_mc, c->stream));
+        if (prof_t)
+            HIP_TRY(c, ldpc::t12_launch_mc(pt->seed, next, (uint32_t)n, mu, sigma, (int)c->cfg.max_iter, c->cfg.beta,
+                                           c->d_queue, c->d_counters, c->d_hist, c->grid_mc, c->stream));
+        else
+            HIP_TRY(c, ldpc::r_launch_mc(rl, pt->seed, next, (uint32_t)n, mu, sigma, c->d_counters, c->d_hist,
+                                         c->stream));
         HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
         HIP_TRY(c, hipMemcpyAsync(cnt, c->d_counters, sizeof(cnt), hipMemcpyDeviceToHost, c->stream));
         HIP_TRY(c, hipMemcpyAsync(h.data(), c->d_hist, nh * sizeof(unsigned long long), hipMemcpyDeviceToHost,
@@ -235,9 +432,13 @@ int ldpc_decode_llrs_device_async(ldpc_ctx *c, const void *d_llr, uint32_t n_cw,
     if (!d_llr || !d_out_llr || !d_out_iters || !d_out_ok) return fail(c, LDPC_ERR_ARG, "null buffer");
     HIP_TRY(c, hipSetDevice(c->device));
     HIP_TRY(c, ldpc::launch_fill_u32(c->d_queue, 1, 0u, c->stream));
-    HIP_TRY(c, ldpc::t12_launch_decode((const uint16_t *)d_llr, (uint16_t *)d_out_llr, (uint32_t *)d_out_iters,
-                                       (uint8_t *)d_out_ok, n_cw, (int)c->cfg.max_iter, c->cfg.beta, c->d_queue,
-                                       c->grid_dec, c->stream));
+    if (c->cfg.profile == LDPC_PROFILE_T)
+        HIP_TRY(c, ldpc::t12_launch_decode((const uint16_t *)d_llr, (uint16_t *)d_out_llr, (uint32_t *)d_out_iters,
+                                           (uint8_t *)d_out_ok, n_cw, (int)c->cfg.max_iter, c->cfg.beta, c->d_queue,
+                                           c->grid_dec, c->stream));
+    else
+        HIP_TRY(c, ldpc::r_launch_decode(c->rl, (const uint16_t *)d_llr, (uint16_t *)d_out_llr,
+                                         (uint32_t *)d_out_iters, (uint8_t *)d_out_ok, n_cw, c->stream));
     return LDPC_OK;
 }
 
@@ -271,7 +472,11 @@ int ldpc_gen_channel(ldpc_ctx *c, uint64_t seed, uint64_t first_cw, uint32_t n_c
     int rc;
     if ((rc = ensure(c, c->out, nb))) return rc;
     HIP_TRY(c, hipSetDevice(c->device));
-    HIP_TRY(c, ldpc::t12_launch_gen(seed, first_cw, n_cw, mu, sigma, (uint16_t *)c->out.p, c->stream));
+    if (c->cfg.profile == LDPC_PROFILE_T)
+        HIP_TRY(c, ldpc::t12_launch_gen(seed, first_cw, n_cw, mu, sigma, (uint16_t *)c->out.p, c->stream));
+    else
+        HIP_TRY(c, ldpc::r_launch_gen(seed, first_cw, n_cw, mu, sigma, l_max_at(c, mu), c->rl.n_tx, c->rl.n,
+                                      (uint16_t *)c->out.p, c->stream));
     HIP_TRY(c, hipMemcpyAsync(out_llr, c->out.p, nb, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     return LDPC_OK;

@@ -18,16 +18,21 @@ LDPC_ERR_NOMEM = -4
 LDPC_ERR_NODEV = -5
 
 LDPC_PROFILE_T = 0
+LDPC_PROFILE_R = 1
+LDPC_PROFILE_R_NMS = 2
 LDPC_CODE_AR4JA = 1
+LDPC_CODE_MATRIX = 2
 
 
 class CodeDesc(C.Structure):
-    _fields_ = [("family", C.c_uint32), ("k_info", C.c_uint32), ("rate_num", C.c_uint32), ("rate_den", C.c_uint32)]
+    _fields_ = [("family", C.c_uint32), ("k_info", C.c_uint32), ("rate_num", C.c_uint32), ("rate_den", C.c_uint32),
+                ("n", C.c_uint32), ("m", C.c_uint32), ("n_punct", C.c_uint32), ("reserved", C.c_uint32),
+                ("row_ptr", C.c_void_p), ("col", C.c_void_p)]
 
 
 class DecoderCfg(C.Structure):
-    _fields_ = [("profile", C.c_uint32), ("max_iter", C.c_uint32), ("beta", C.c_float),
-                ("reserved", C.c_uint32 * 5)]
+    _fields_ = [("profile", C.c_uint32), ("max_iter", C.c_uint32), ("beta", C.c_float), ("l_max", C.c_float),
+                ("r_max", C.c_float), ("l_post_max", C.c_float), ("gamma", C.c_float), ("reserved", C.c_uint32)]
 
 
 class CodeInfo(C.Structure):
@@ -51,6 +56,7 @@ SIGNATURES = {
     "ldpc_last_error": (C.c_char_p, [C.c_void_p]),
     "ldpc_get_code_info": (C.c_int, [C.c_void_p, C.POINTER(CodeInfo)]),
     "ldpc_get_schedule": (C.c_int, [C.POINTER(CodeDesc), C.c_void_p, C.c_int32]),
+    "ldpc_row_levels": (C.c_int, [C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]),
     "ldpc_run_point": (C.c_int, [C.c_void_p, C.POINTER(PointCfg), C.POINTER(PointResult), C.c_void_p]),
     "ldpc_decode_llrs": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]),
     "ldpc_decode_llrs_device_async": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p,

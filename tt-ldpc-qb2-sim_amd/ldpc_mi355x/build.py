@@ -19,7 +19,7 @@ CSRC = os.path.join(ROOT, "csrc")
 LIB = os.path.join(PKG, "libldpc.so")
 ARCH = os.environ.get("LDPC_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["decode_t.hip", "runtime.cpp"]
+SOURCES = ["decode_t.hip", "decode_r.hip", "runtime.cpp"]
 HEADERS = ["ar4ja_schedule.h", "device_math.h", "kernels.h"]
 
 HIPCC_FLAGS = [

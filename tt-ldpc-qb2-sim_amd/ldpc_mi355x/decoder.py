@@ -57,6 +57,19 @@ class PointResult:
         return self.blocks * self.n_tx / self.kernel_s / 1e9 if self.kernel_s else 0.0
 
 
+def row_levels(rows, n):
+    """Profile R levels of rows (1-based), as libldpc computes them."""
+    lib = L.load()
+    rp = np.zeros(len(rows) + 1, dtype=np.int32)
+    rp[1:] = np.cumsum([len(r) for r in rows])
+    col = np.concatenate([np.asarray(r, dtype=np.int32) for r in rows]).astype(np.int32)
+    out = np.zeros(len(rows), dtype=np.int32)
+    nl = lib.ldpc_row_levels(n, len(rows), rp.ctypes.data, col.ctypes.data, out.ctypes.data)
+    if nl < 0:
+        raise L.LdpcError(nl, "invalid matrix")
+    return out
+
+
 def compiled_schedule(k_info: int = 1024, rate: str = "1/2"):
     """Layer schedule compiled into the kernels: [(first_row, [(base, shift), ...]), ...]."""
     lib = L.load()
@@ -74,15 +87,38 @@ def compiled_schedule(k_info: int = 1024, rate: str = "1/2"):
     return out
 
 
+PROFILES = {"T": L.LDPC_PROFILE_T, "R": L.LDPC_PROFILE_R, "R_NMS": L.LDPC_PROFILE_R_NMS}
+
+
 class Engine:
-    """One libldpc context: a code, a decoder configuration and a device."""
+    """One libldpc context: a code, a decoder configuration and a device.
+
+    Profiles:
+      "T"     the golden stepper's tree Approx-Min* (AR4JA r1/2 K=1024);
+      "R"     the scalar_5w recursive Approx-Min* that run_simulation.py drives;
+      "R_NMS" the same engine's normalized min-sum (its -n flag).
+    A Profile R engine takes either an AR4JA (k_info, rate) code in makeH row
+    order or an explicit matrix: ``matrix=(rows, n)`` with rows in decoding
+    (file) order, and ``n_punct`` punctured trailing variables.
+    """
 
     def __init__(self, k_info: int = 1024, rate: str = "1/2", max_iter: int = 200, beta: float = 0.9375,
-                 profile: int = L.LDPC_PROFILE_T, device: int = 0):
+                 profile="T", device: int = 0, matrix=None, n_punct: int = 0, l_max: float = 0.0,
+                 r_max: float = 0.0, l_post_max: float = 0.0, gamma: float = 0.0):
         self._lib = L.load()
-        num, den = RATE_FRACTIONS[rate]
-        desc = L.CodeDesc(L.LDPC_CODE_AR4JA, k_info, num, den)
-        cfg = L.DecoderCfg(profile, max_iter, beta)
+        self._h = None
+        prof = PROFILES[profile] if isinstance(profile, str) else int(profile)
+        if matrix is not None:
+            rows, n = matrix
+            rp = np.zeros(len(rows) + 1, dtype=np.int32)
+            rp[1:] = np.cumsum([len(r) for r in rows])
+            col = np.concatenate([np.asarray(r, dtype=np.int32) for r in rows]).astype(np.int32)
+            desc = L.CodeDesc(L.LDPC_CODE_MATRIX, 0, 0, 0, n, len(rows), n_punct, 0, rp.ctypes.data,
+                              col.ctypes.data)
+        else:
+            num, den = RATE_FRACTIONS[rate]
+            desc = L.CodeDesc(L.LDPC_CODE_AR4JA, k_info, num, den)
+        cfg = L.DecoderCfg(prof, max_iter, beta, l_max, r_max, l_post_max, gamma)
         h = C.c_void_p()
         rc = self._lib.ldpc_create(C.byref(desc), C.byref(cfg), device, C.byref(h))
         if rc != L.LDPC_OK:
@@ -92,7 +128,7 @@ class Engine:
         self._check(self._lib.ldpc_get_code_info(self._h, C.byref(info)))
         self.n, self.m, self.k_info, self.n_tx = info.n, info.m, info.k_info, info.n_tx
         self.z, self.n_layers, self.n_edges = info.z, info.n_layers, info.n_edges
-        self.max_iter, self.beta, self.rate = max_iter, beta, rate
+        self.max_iter, self.beta, self.rate, self.profile = max_iter, beta, rate, prof
 
     def _check(self, rc: int) -> None:
         if rc != L.LDPC_OK:
@@ -123,6 +159,7 @@ class Engine:
 
     def schedule(self):
         return compiled_schedule(self.k_info, self.rate)
+
 
     @property
     def stream(self) -> int:

@@ -20,6 +20,8 @@ LIB = os.path.join(PKG, "libldpc.so")
 ARCH = os.environ.get("LDPC_OFFLOAD_ARCH", "gfx950")
 
 SOURCES = ["decode_t.hip", "decode_r.hip", "runtime.cpp"]
+# command-line shims speaking the reference engines' CLI, linked to libldpc
+CLIS = {"ldpc_sim_scalar": "cli_scalar.cpp", "ldpc_sim_circulant": "cli_circulant.cpp"}
 HEADERS = ["ar4ja_schedule.h", "device_math.h", "kernels.h"]
 
 HIPCC_FLAGS = [
@@ -49,7 +51,19 @@ def build(force: bool = True, verbose: bool = False) -> str:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True, cwd=CSRC)
     os.replace(tmp, LIB)
+    build_clis(verbose)
     return LIB
+
+
+def build_clis(verbose: bool = False) -> None:
+    inc = os.path.join(os.path.dirname(ROOT), "include")
+    for exe, src in CLIS.items():
+        cmd = ["g++", "-O2", "-std=c++17", "-Wall", f"-I{inc}", os.path.join(CSRC, src), f"-L{PKG}", "-lldpc",
+               "-L/opt/rocm/lib", "-Wl,-rpath-link,/opt/rocm/lib", "-Wl,-rpath,$ORIGIN", "-o",
+               os.path.join(PKG, exe)]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
 
 
 if __name__ == "__main__":

@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define LDPC_ABI_VERSION 1
+#define LDPC_ABI_VERSION 2
 
 typedef enum ldpc_status {
     LDPC_OK = 0,
@@ -44,8 +44,10 @@ typedef enum ldpc_status {
 enum {
     /* Tree Approximate-Min* with bf16 rounding of q, r and L, truncated
      * constants and damping beta: the reference's golden stepper
-     * (tests/test_circulant_dma_stepper.cpp:107-327, :743-815).
-     * AR4JA r1/2 K=1024 only. */
+     * (tests/test_circulant_dma_stepper.cpp:107-327, :743-815).  The nine
+     * AR4JA codes; degree-10 and degree-18 checks (rates 2/3, 4/5) use the
+     * degree-6 tree generalised by pairs (j, j + d/2) and a prefix/suffix
+     * pass over the pairs, which the reference does not define. */
     LDPC_PROFILE_T = 0,
     /* Recursive Approximate-Min* of the scalar_5w engine that
      * run_simulation.py drives (engines/scalar_5w/ldpc_decoder_core.h:
@@ -88,8 +90,19 @@ typedef struct ldpc_decoder_cfg {
     float r_max;       /* |check message| <= r_max */
     float l_post_max;  /* |posterior LLR| <= l_post_max */
     float gamma;       /* if > 0: l_max = gamma * mu at each Eb/N0 */
-    uint32_t reserved;
+    uint32_t variant;  /* Profile T kernel variant: 0 = library default, or
+                          LDPC_VARIANT(checks per thread, layer order) */
 } ldpc_decoder_cfg;
+
+/* Profile T kernel variants.  checks_per_thread (1 or 2) sets how many checks
+ * of a layer one GPU thread updates (threads per codeword = z / checks);
+ * layer_order 0 decodes block rows 1, 2, 0 (the golden's tile order,
+ * stepper:765-771), 1 decodes block rows 0, 1, 2.  The checks-per-thread
+ * choice never changes results; the layer order does.  0 selects the
+ * specialised kernel for AR4JA r1/2 K=1024 and a tuned variant with layer
+ * order 0 for the other codes. */
+#define LDPC_VARIANT(checks_per_thread, layer_order) \
+    (0x100u | ((uint32_t)(checks_per_thread) & 0xFu) | (((uint32_t)(layer_order) & 0xFu) << 4))
 
 typedef struct ldpc_code_info {
     uint32_t n;       /* variables (transmitted + punctured) */
@@ -138,11 +151,16 @@ const char *ldpc_last_error(const ldpc_ctx *ctx);
 
 int ldpc_get_code_info(const ldpc_ctx *ctx, ldpc_code_info *info);
 
-/* Layer schedule compiled into the kernels for a code (needs no GPU): per
- * layer [first_row, deg, (base, shift) * deg], layers in decoding order;
- * variable of check j of a layer = base + (shift + j) mod z.  Returns the
- * number of int32 written or a negative status. */
-int ldpc_get_schedule(const ldpc_code_desc *code, int32_t *out, int32_t cap);
+/* Layer schedule compiled into the Profile T kernels for an AR4JA code and
+ * layer order (0 or 1, see LDPC_VARIANT; needs no GPU): per layer
+ * [first_row, deg, (base, shift) * deg], layers in decoding order; variable
+ * of check j of a layer = base + (shift + j) mod z.  Returns the number of
+ * int32 written or a negative status. */
+int ldpc_get_schedule(const ldpc_code_desc *code, uint32_t layer_order, int32_t *out, int32_t cap);
+
+/* Profile T kernel variants built for an AR4JA code (needs no GPU): writes up
+ * to cap LDPC_VARIANT values and returns how many exist. */
+int ldpc_list_variants(const ldpc_code_desc *code, uint32_t *out, int32_t cap);
 
 /* Level of every row for the Profile R schedule (needs no GPU): level(i) =
  * 1 + max level of the earlier rows sharing a variable with row i.  Rows of
@@ -183,6 +201,53 @@ int ldpc_gen_channel(ldpc_ctx *ctx, uint64_t seed, uint64_t first_cw, uint32_t n
  * R = k_info / n_tx:  mu = 4 R g, sigma = sqrt(8 R g), g = 10^(ebn0/10)
  * (ldpc_sim_scalar.cpp:352-355). */
 int ldpc_channel_params(const ldpc_ctx *ctx, double ebn0_db, float *mu, float *sigma);
+
+/* One variant's measurement by ldpc_autotune. */
+typedef struct ldpc_tune_result {
+    uint32_t variant;
+    uint32_t threads_per_wg;   /* workgroup size */
+    uint32_t cw_per_wg;        /* codewords in flight per workgroup */
+    uint32_t wg_per_cu;        /* resident workgroups per CU */
+    double gsps;               /* transmitted symbols per second of kernel time */
+    double mean_iterations;
+    uint64_t frame_errors;
+    uint64_t kernel_ns;
+} ldpc_tune_result;
+
+/* Autotuner: run the same n_cw codewords (seed, codeword ids 0..n_cw-1) at
+ * ebn0_db through every Profile T variant of an AR4JA code, best of `reps`
+ * timed launches each, and write one ldpc_tune_result per variant (at most
+ * cap).  Variants with the same layer order must give identical counters
+ * (checked; a mismatch is LDPC_ERR_HIP).  *best receives the fastest variant
+ * among those with layer order 0 if any_order is 0, among all otherwise.
+ * Returns the number of results.  No reference counterpart: the reference
+ * has one hand-tuned kernel per engine. */
+int ldpc_autotune(const ldpc_code_desc *code, const ldpc_decoder_cfg *cfg, int device, double ebn0_db, uint64_t seed,
+                  uint32_t n_cw, uint32_t reps, int any_order, ldpc_tune_result *out, int32_t cap, uint32_t *best);
+
+/* Mixed-code batch: several AR4JA codes (Profile T), each with its own kernel
+ * variant, Eb/N0 and codeword stream, run as one captured HIP graph per
+ * round: every code's persistent kernel takes codewords from its own device
+ * queue; a round simulates up to pts[i].chunk_cw codewords of every code
+ * that has not reached its max_blocks / target_fe.  Replaces running the
+ * reference's engine once per code (run_simulation.py:164-256 per matrix). */
+typedef struct ldpc_batch ldpc_batch;
+
+/* variants may be NULL (library defaults); cfg->variant is ignored. */
+int ldpc_batch_create(const ldpc_code_desc *codes, const uint32_t *variants, uint32_t n_codes,
+                      const ldpc_decoder_cfg *cfg, int device, ldpc_batch **out);
+
+/* pts, res: n_codes entries; hist (may be NULL): n_codes * (max_iter + 1),
+ * laid out code by code as in ldpc_run_point.  Results ADD to res and hist.
+ * res[i].kernel_ns is the graph time of the rounds code i took part in. */
+int ldpc_batch_run(ldpc_batch *b, const ldpc_point_cfg *pts, ldpc_point_result *res, uint64_t *hist);
+
+/* Code info of batch member i. */
+int ldpc_batch_code_info(const ldpc_batch *b, uint32_t i, ldpc_code_info *info);
+
+const char *ldpc_batch_last_error(const ldpc_batch *b);
+
+void ldpc_batch_destroy(ldpc_batch *b);
 
 /* Wait for all work on the context's stream. */
 int ldpc_sync(ldpc_ctx *ctx);

@@ -91,32 +91,63 @@ EXPORT float orc_amin_t(float a, float b) {
     return smax(mn - corr, 0.0f);
 }
 
+/* Tree min* of the other edges for an even degree D = 2h, generalising the
+ * golden degree-6 tree (stepper:151-225) to the degree-10 and degree-18
+ * checks of the rate-2/3 and rate-4/5 AR4JA codes, which the reference's
+ * Profile T never decodes (its golden and silicon paths are rate 1/2 only):
+ * edge j is paired with edge j + h; p_j = A(m_j, m_{j+h}); the combination of
+ * all pairs but j is formed from a prefix P_j = A(P_{j-1}, p_j) and a suffix
+ * S_j = A(p_j, S_{j+1}) as o_j = A(P_{j-1}, S_{j+1}) (o_0 = S_1,
+ * o_{h-1} = P_{h-2}); then r_j = A(o_j, m_{j+h}) beta and
+ * r_{j+h} = A(o_j, m_j) beta.  For h = 3 these are exactly the golden's
+ * o0 = A(p14, p25), o1 = A(p03, p25), o2 = A(p03, p14) (A is symmetric), so
+ * degree 6 is the golden tree.                                              */
+EXPORT void orc_tree_even(const float *m, int h, float beta, float *r) {
+    float p[9], P[9], S[9];
+    for (int j = 0; j < h; ++j) p[j] = orc_amin_t(m[j], m[j + h]);
+    P[0] = p[0];
+    for (int j = 1; j < h - 1; ++j) P[j] = orc_amin_t(P[j - 1], p[j]);
+    S[h - 1] = p[h - 1];
+    for (int j = h - 2; j >= 1; --j) S[j] = orc_amin_t(p[j], S[j + 1]);
+    for (int j = 0; j < h; ++j) {
+        float o = j == 0 ? S[1] : j == h - 1 ? P[h - 2] : orc_amin_t(P[j - 1], S[j + 1]);
+        r[j] = orc_amin_t(o, m[j + h]) * beta;
+        r[j + h] = orc_amin_t(o, m[j]) * beta;
+    }
+}
+
+/* The golden degree-6 tree as the stepper writes it (stepper:151-225). */
+EXPORT void orc_tree6(const float *m, float beta, float *r) {
+    float p03 = orc_amin_t(m[0], m[3]), p14 = orc_amin_t(m[1], m[4]), p25 = orc_amin_t(m[2], m[5]);
+    float o0 = orc_amin_t(p14, p25);          /* :174 */
+    float o1 = orc_amin_t(p03, p25);          /* :199 */
+    float o2 = orc_amin_t(p03, p14);          /* :224 */
+    r[0] = orc_amin_t(o0, m[3]) * beta;  r[3] = orc_amin_t(o0, m[0]) * beta;
+    r[1] = orc_amin_t(o1, m[4]) * beta;  r[4] = orc_amin_t(o1, m[1]) * beta;
+    r[2] = orc_amin_t(o2, m[5]) * beta;  r[5] = orc_amin_t(o2, m[2]) * beta;
+}
+
+#define ORC_MAX_DEG_T 18
+
 /* One Profile T check update, in place on L (per variable) and R (per edge).
- * deg is 3 or 6; v[] are the check's variables in ascending order; iter0
- * selects q = L (R treated as 0, stepper:346-348 and :522-531).             */
+ * deg is 3 or even (4..18); v[] are the check's variables in ascending order;
+ * iter0 selects q = L (R treated as 0, stepper:346-348 and :522-531).      */
 static void check_update_t(uint16_t *L, uint16_t *R, const int32_t *v, int deg, int iter0, float beta) {
-    float q[6] = {0, 0, 0, 0, 0, 0};
+    float q[ORC_MAX_DEG_T], m[ORC_MAX_DEG_T], r[ORC_MAX_DEG_T];
     uint32_t s = 0;
     for (int d = 0; d < deg; ++d) {
         float l = bf2f(L[v[d]]);
         q[d] = iter0 ? l : rne(l - bf2f(R[d]));   /* compute_tile_deg6_sfpi:112-116 */
         s ^= f2u(q[d]);                             /* sign parity, :161 */
+        m[d] = fabsf(q[d]);
     }
-    float r[6];
     if (deg == 6) {
-        float m0 = fabsf(q[0]), m1 = fabsf(q[1]), m2 = fabsf(q[2]);
-        float m3 = fabsf(q[3]), m4 = fabsf(q[4]), m5 = fabsf(q[5]);
-        float p03 = orc_amin_t(m0, m3), p14 = orc_amin_t(m1, m4), p25 = orc_amin_t(m2, m5);
-        float o0 = orc_amin_t(p14, p25);          /* :174 */
-        float o1 = orc_amin_t(p03, p25);          /* :199 */
-        float o2 = orc_amin_t(p03, p14);          /* :224 */
-        r[0] = orc_amin_t(o0, m3) * beta;  r[3] = orc_amin_t(o0, m0) * beta;
-        r[1] = orc_amin_t(o1, m4) * beta;  r[4] = orc_amin_t(o1, m1) * beta;
-        r[2] = orc_amin_t(o2, m5) * beta;  r[5] = orc_amin_t(o2, m2) * beta;
-    } else {
-        float m0 = fabsf(q[0]), m1 = fabsf(q[1]), m2 = fabsf(q[2]);
-        r[0] = orc_amin_t(m1, m2);  r[1] = orc_amin_t(m0, m2);  r[2] = orc_amin_t(m0, m1);
+        orc_tree6(m, beta, r);
+    } else if (deg == 3) {
+        r[0] = orc_amin_t(m[1], m[2]);  r[1] = orc_amin_t(m[0], m[2]);  r[2] = orc_amin_t(m[0], m[1]);
         if (beta < 1.0f) { r[0] *= beta; r[1] *= beta; r[2] *= beta; }   /* :300-304 */
+    } else {
+        orc_tree_even(m, deg / 2, beta, r);
     }
     for (int d = 0; d < deg; ++d) {
         /* copysgn(r, s ^ q_d) then RNE (sfpi_host_emu.h:466-476; stepper:178-186) */
@@ -125,6 +156,17 @@ static void check_update_t(uint16_t *L, uint16_t *R, const int32_t *v, int deg, 
         R[d] = f2bf(out);
         L[v[d]] = f2bf(q[d] + out);
     }
+}
+
+/* One check update on explicit values (unit tests of the tree): L_in/R_in
+ * are the check's deg variable and message values; results in L_out/R_out. */
+EXPORT int orc_check_update_t(int deg, const uint16_t *L_in, const uint16_t *R_in, int iter0, float beta,
+                              uint16_t *L_out, uint16_t *R_out) {
+    if (deg != 3 && (deg < 4 || deg > ORC_MAX_DEG_T || (deg & 1))) return -1;
+    int32_t v[ORC_MAX_DEG_T];
+    for (int d = 0; d < deg; ++d) { v[d] = d; L_out[d] = L_in[d]; R_out[d] = R_in[d]; }
+    check_update_t(L_out, R_out, v, deg, iter0, beta);
+    return 0;
 }
 
 /* Syndrome over all checks with hard bit = (bf16 < 0.0f) (stepper:773-796). */
@@ -166,7 +208,7 @@ static int decode_one_t(uint16_t *L, uint16_t *R, int m, const int32_t *rp, cons
 static int check_degrees_t(int m, const int32_t *rp) {
     for (int i = 0; i < m; ++i) {
         int d = rp[i + 1] - rp[i];
-        if (d != 3 && d != 6) return -1;
+        if (d != 3 && (d < 4 || d > ORC_MAX_DEG_T || (d & 1))) return -1;
     }
     return 0;
 }

@@ -76,6 +76,13 @@ def lib():
                                               C.c_int, u16p]
         L.orc_num_threads.restype = C.c_int
         L.orc_num_threads.argtypes = []
+        f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+        L.orc_tree_even.restype = None
+        L.orc_tree_even.argtypes = [f32p, C.c_int, C.c_float, f32p]
+        L.orc_tree6.restype = None
+        L.orc_tree6.argtypes = [f32p, C.c_float, f32p]
+        L.orc_check_update_t.restype = C.c_int
+        L.orc_check_update_t.argtypes = [C.c_int, u16p, u16p, C.c_int, C.c_float, u16p, u16p]
         _lib = L
     return _lib
 
@@ -93,6 +100,32 @@ def amin_t(a: float, b: float) -> float:
     return lib().orc_amin_t(a, b)
 
 
+def tree_even(m, beta: float = 0.9375) -> np.ndarray:
+    """Generalised even-degree tree (pairs j, j+h; prefix/suffix over pairs)."""
+    m = np.ascontiguousarray(m, dtype=np.float32)
+    r = np.empty_like(m)
+    lib().orc_tree_even(m, len(m) // 2, beta, r)
+    return r
+
+
+def tree6(m, beta: float = 0.9375) -> np.ndarray:
+    """The golden degree-6 tree as the reference stepper writes it."""
+    m = np.ascontiguousarray(m, dtype=np.float32)
+    r = np.empty_like(m)
+    lib().orc_tree6(m, beta, r)
+    return r
+
+
+def check_update_t(L_in, R_in, iter0: bool = False, beta: float = 0.9375):
+    """One Profile T check update on explicit bf16 values: (L_out, R_out)."""
+    L_in = np.ascontiguousarray(L_in, dtype=np.uint16)
+    R_in = np.ascontiguousarray(R_in, dtype=np.uint16)
+    Lo, Ro = np.empty_like(L_in), np.empty_like(R_in)
+    if lib().orc_check_update_t(len(L_in), L_in, R_in, int(iter0), beta, Lo, Ro):
+        raise ValueError("unsupported degree")
+    return Lo, Ro
+
+
 def decode_t(code, llr: np.ndarray, max_iter: int = 200, beta: float = 0.9375, threads: int = 0):
     llr = np.ascontiguousarray(llr, dtype=np.uint16).reshape(-1, code.n)
     n_cw = llr.shape[0]
@@ -102,7 +135,7 @@ def decode_t(code, llr: np.ndarray, max_iter: int = 200, beta: float = 0.9375, t
     ok = np.empty(n_cw, dtype=np.uint8)
     rc = lib().orc_decode_t(code.n, code.m, rp, col, sched, n_cw, llr, max_iter, beta, out, iters, ok, threads)
     if rc:
-        raise ValueError("oracle Profile T supports check degrees 3 and 6 only")
+        raise ValueError("oracle Profile T supports check degree 3 and even degrees 4..18")
     return out, iters, ok
 
 
@@ -162,7 +195,7 @@ def run_point_t(code, seed: int, first_cw: int, n_cw: int, mu: float, sigma: flo
     rc = lib().orc_run_point_t(code.n, code.m, rp, col, sched, code.n_tx, code.k, seed, first_cw, n_cw, mu, sigma,
                                max_iter, beta, threads, cnt, hist)
     if rc:
-        raise ValueError("oracle Profile T supports check degrees 3 and 6 only")
+        raise ValueError("oracle Profile T supports check degree 3 and even degrees 4..18")
     return dict(blocks=int(cnt[0]), frame_errors=int(cnt[1]), bit_errors=int(cnt[2]), iter_sum=int(cnt[3]),
                 hist=hist)
 

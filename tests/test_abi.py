@@ -40,6 +40,7 @@ def test_struct_layouts_match_header():
     assert C.sizeof(L.CodeInfo) == 32
     assert C.sizeof(L.PointCfg) == 48
     assert C.sizeof(L.PointResult) == 64
+    assert C.sizeof(L.TuneResult) == 48
 
 
 def test_create_rejects_bad_arguments_without_gpu_work():
@@ -73,11 +74,62 @@ def test_compiled_schedule_equals_python_builder(code12):
     assert got == exp
 
 
-def test_schedule_of_unbuilt_code_is_unsupported():
+@pytest.mark.parametrize("order", [0, 1])
+def test_compiled_schedules_of_all_nine_codes_equal_python_builder(order):
+    """The kernels' compile-time schedules (ar4ja_schedule.h) equal codes.py's
+    construction from the makeH tables, for both layer orders."""
+    from ldpc_mi355x import codes, decoder
+    br = codes.BLOCK_ROW_ORDER_T if order == 0 else (0, 1, 2)
+    for k, r in decoder.AR4JA_CODES:
+        code = codes.ar4ja(k, r, block_row_order=br)
+        got = decoder.compiled_schedule(k, r, order)
+        assert got == [(f, [(e.base, e.shift) for e in es]) for f, es in code.layers], (k, r, order)
+
+
+def test_schedule_rejects_other_codes_and_orders():
     lib = L.load()
     buf = (C.c_int32 * 64)()
-    desc = L.CodeDesc(L.LDPC_CODE_AR4JA, 4096, 2, 3)
-    assert lib.ldpc_get_schedule(C.byref(desc), buf, 64) == L.LDPC_ERR_UNSUPPORTED
+    desc = L.CodeDesc(L.LDPC_CODE_AR4JA, 1000, 1, 2)
+    assert lib.ldpc_get_schedule(C.byref(desc), 0, buf, 64) == L.LDPC_ERR_UNSUPPORTED
+    ok = L.CodeDesc(L.LDPC_CODE_AR4JA, 4096, 2, 3)
+    assert lib.ldpc_get_schedule(C.byref(ok), 2, buf, 64) == L.LDPC_ERR_ARG
+    assert lib.ldpc_get_schedule(C.byref(ok), 0, buf, 8) == L.LDPC_ERR_ARG  # too small
+
+
+def test_variant_tables():
+    """Every code has C = 1 and 2 in both layer orders, except K=16384 r1/2
+    (z = 2048 > 1024 threads per workgroup: C = 2 only)."""
+    from ldpc_mi355x import decoder
+    for k, r in decoder.AR4JA_CODES:
+        vs = decoder.list_variants(k, r)
+        fields = sorted(L.variant_fields(v) for v in vs)
+        if (k, r) == (16384, "1/2"):
+            assert fields == [(2, 0), (2, 1)]
+        else:
+            assert fields == [(1, 0), (1, 1), (2, 0), (2, 1)]
+        assert all(v == L.variant(*L.variant_fields(v)) for v in vs)
+
+
+def test_variant_and_batch_arguments_checked_without_gpu():
+    lib = L.load()
+    h = C.c_void_p()
+    code = L.CodeDesc(L.LDPC_CODE_AR4JA, 4096, 4, 5)
+    cfg = L.DecoderCfg(L.LDPC_PROFILE_T, 200, 0.9375, 0, 0, 0, 0, L.variant(3, 0))
+    assert lib.ldpc_create(C.byref(code), C.byref(cfg), 0, C.byref(h)) == L.LDPC_ERR_UNSUPPORTED
+    cfg = L.DecoderCfg(L.LDPC_PROFILE_R, 20, 0.0, 0, 0, 0, 0, L.variant(1, 0))
+    assert lib.ldpc_create(C.byref(code), C.byref(cfg), 0, C.byref(h)) == L.LDPC_ERR_ARG
+    descs = (L.CodeDesc * 2)(L.CodeDesc(L.LDPC_CODE_AR4JA, 1024, 1, 2), L.CodeDesc(L.LDPC_CODE_AR4JA, 1000, 1, 2))
+    cfg_t = L.DecoderCfg(L.LDPC_PROFILE_T, 200, 0.9375)
+    assert lib.ldpc_batch_create(descs, None, 2, C.byref(cfg_t), 0, C.byref(h)) == L.LDPC_ERR_ARG
+    assert b"batch member 1" in lib.ldpc_batch_last_error(None)
+    cfg_r = L.DecoderCfg(L.LDPC_PROFILE_R, 20, 0.0)
+    assert lib.ldpc_batch_create(descs, None, 1, C.byref(cfg_r), 0, C.byref(h)) == L.LDPC_ERR_UNSUPPORTED
+    best = C.c_uint32()
+    out = (L.TuneResult * 4)()
+    assert lib.ldpc_autotune(C.byref(descs[0]), C.byref(cfg_r), 0, 2.0, 1, 64, 1, 0, out, 4, C.byref(best)) \
+        == L.LDPC_ERR_ARG
+    assert lib.ldpc_batch_run(None, None, None, None) == L.LDPC_ERR_ARG
+    lib.ldpc_batch_destroy(None)
 
 
 def test_row_levels_rejects_bad_matrices():

@@ -10,8 +10,9 @@
 //  engines/circulant_dma_4cw/ar4ja_circulant_descriptors_4cw.h:43-1016).
 //
 // Construction: matrices/makeH_AR4JA.m:74-102 (theta/phi tables :19-72).
-// Block-row order 1, 2, 0 is the golden's tile order (stepper:765-771: tiles
-// 0..31 cover rows M..3M-1, tiles 32..39 rows 0..M-1).
+// Block-row order 1, 2, 0 (order_id 0) is the golden's tile order
+// (stepper:765-771: tiles 0..31 cover rows M..3M-1, tiles 32..39 rows
+// 0..M-1); order_id 1 decodes block rows 0, 1, 2 (the makeH row order).
 #pragma once
 #include <stdint.h>
 
@@ -97,7 +98,7 @@ constexpr int ilog2(int x) {
     return r;
 }
 
-constexpr Schedule make_schedule(int k_id, int rate_id) {
+constexpr Schedule make_schedule(int k_id, int rate_id, int order_id = 0) {
     Schedule s{};
     const int M = block_size(k_id, rate_id), Z = M / 4, t = ilog2(M) - 7;
     s.M = M;
@@ -108,10 +109,10 @@ constexpr Schedule make_schedule(int k_id, int rate_id) {
     s.k_info = 1024 << (2 * k_id);
     s.n_layers = 12;
     s.n_edges = 0;
-    const int order[3] = {1, 2, 0};
+    const int order[2][3] = {{1, 2, 0}, {0, 1, 2}};
     int li = 0;
     for (int oi = 0; oi < 3; ++oi) {
-        const int br = order[oi];
+        const int br = order[order_id][oi];
         for (int j = 0; j < 4; ++j) {
             Layer L{};
             L.first_row = br * M + j * Z;

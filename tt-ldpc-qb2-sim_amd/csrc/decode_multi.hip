@@ -1,0 +1,561 @@
+// decode_multi.hip -- Profile T decoder for the nine CCSDS AR4JA codes on
+// gfx950: K = 1024, 4096, 16384 at rates 1/2, 2/3, 4/5 (circulant M = 128 ..
+// 8192; check degrees 3 and 6, 10 or 18).
+//
+// Compiled once per code: -DLDPC_KID=0..2 (K = 1024 << 2 KID) and
+// -DLDPC_RID=0..2 (rate 1/2, 2/3, 4/5).  Each object instantiates the kernel
+// for C = 1 or 2 checks per thread and for two layer orders, and exports its
+// variant table (multi_table_k<KID>_r<RID>).
+//
+// Arithmetic: the golden's Profile T check update (stepper:107-327: q =
+// RNE(L - R), tree Approx-Min* with bf16-truncated constants, damping beta,
+// copysign, RNE of R' and L').  The degree-3 and degree-6 trees are the
+// golden's; even degrees D > 6 (the rate-2/3 and rate-4/5 codes, which the
+// reference's Profile T never decodes) pair edge j with edge j + D/2 and
+// combine the other pairs through a prefix/suffix pass, which at D = 6 is
+// exactly the golden's tree.  Syndrome with hard bit = (bf16 < 0) after every
+// iteration; a converged codeword is latched; frame error <=> not converged;
+// bit errors = information bits <= 0 of failed codewords (stepper:773-815).
+//
+// Mapping.  A layer is Z = M/4 checks of one block-row quarter (no two share
+// a variable).  A codeword is owned by G = Z / C threads; thread t updates
+// checks t and t + G (C = 2) or check t (C = 1) of every layer.
+//  * G <= 64 ("wave mode"): a wave holds 64 / G codeword slots in lockstep.
+//    LDS requests of one wave complete in issue order, so a compiler fence
+//    separates layers.  After each iteration every slot tests its own
+//    syndrome; a slot whose codeword converged or ran out of iterations is
+//    tallied and refilled from the work queue while the other slots go on.
+//  * G > 64 ("group mode"): a workgroup of G threads owns one codeword;
+//    workgroup barriers separate layers.
+// A codeword's LLRs live in LDS (N bf16); its messages R live in VGPRs, two
+// bf16 per dword (the two checks of a thread for C = 2, two edges of the
+// check for C = 1).
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <utility>
+
+#include "ar4ja_schedule.h"
+#include "device_math.h"
+#include "kernels.h"
+
+#pragma clang fp contract(off)
+
+#if !defined(LDPC_KID) || !defined(LDPC_RID)
+#error "compile with -DLDPC_KID=<0|1|2> -DLDPC_RID=<0|1|2>"
+#endif
+
+namespace ldpc {
+namespace {
+
+constexpr int HIST_LDS = 256;
+constexpr int NLAYERS = 12;
+
+__device__ __forceinline__ float ld_bf(const uint16_t *p) { return u2f((uint32_t)(*p) << 16); }
+__device__ __forceinline__ void wave_fence() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront"); }
+// Hard decision "bf16 < 0.0f" (stepper:781): sign set and magnitude nonzero.
+__device__ __forceinline__ uint32_t hard_neg(uint32_t h) { return h > 0x8000u ? 1u : 0u; }
+// Bit error test "bf16 <= 0.0f" (stepper:806).
+__device__ __forceinline__ uint32_t nonpos(uint32_t h) { return ((h & 0x8000u) || h == 0u) ? 1u : 0u; }
+
+// Tree min* over the other edges of a check, times beta.
+template <int D>
+__device__ __forceinline__ void tree(const float (&m)[D], float beta, float (&r)[D]) {
+    if constexpr (D == 3) {
+        r[0] = __fmul_rn(amin_t(m[1], m[2]), beta);
+        r[1] = __fmul_rn(amin_t(m[0], m[2]), beta);
+        r[2] = __fmul_rn(amin_t(m[0], m[1]), beta);
+    } else {
+        static_assert(D % 2 == 0 && D >= 4, "Profile T: degree 3 or even");
+        constexpr int H = D / 2;
+        float p[H], P[H], S[H];
+#pragma unroll
+        for (int j = 0; j < H; ++j) p[j] = amin_t(m[j], m[j + H]);
+        P[0] = p[0];
+#pragma unroll
+        for (int j = 1; j < H - 1; ++j) P[j] = amin_t(P[j - 1], p[j]);
+        S[H - 1] = p[H - 1];
+#pragma unroll
+        for (int j = H - 2; j >= 1; --j) S[j] = amin_t(p[j], S[j + 1]);
+#pragma unroll
+        for (int j = 0; j < H; ++j) {
+            const float o = j == 0 ? S[1] : j == H - 1 ? P[H - 2] : amin_t(P[j - 1], S[j + 1]);
+            r[j] = __fmul_rn(amin_t(o, m[j + H]), beta);
+            r[j + H] = __fmul_rn(amin_t(o, m[j]), beta);
+        }
+    }
+}
+
+template <int C_, int ORD_>
+struct Cfg {
+    static constexpr int C = C_, ORD = ORD_;
+    static constexpr ar4ja::Schedule S = ar4ja::make_schedule(LDPC_KID, LDPC_RID, ORD_);
+    static constexpr int Z = S.Z, N = S.N, NP = S.n_punct, NTX = S.N - S.n_punct, K = S.k_info;
+    static constexpr int G = Z / C;  // threads per codeword
+    static constexpr bool WAVE = G <= 64;
+    static constexpr int SLOTS = WAVE ? 64 / G : 1;  // codewords per wave (wave mode)
+    static constexpr int WPW = WAVE ? 4 : G / 64;    // waves per workgroup
+    static constexpr int THREADS = 64 * WPW;
+    static constexpr int CW_PER_WG = WAVE ? SLOTS * WPW : 1;
+    static constexpr int rdw(int deg) { return C == 2 ? deg : (deg + 1) / 2; }
+    static constexpr int r_off(int li) {
+        int o = 0;
+        for (int i = 0; i < li; ++i) o += rdw(S.layer[i].deg);
+        return o;
+    }
+    static constexpr int NR = r_off(NLAYERS);
+    static_assert(G >= 16 && G <= 1024 && (G & (G - 1)) == 0, "threads per codeword");
+    static_assert(NTX % 4 == 0 && NP % 4 == 0 && N % 8 == 0, "channel layout");
+};
+
+// One layer of the thread's checks; Ls is the codeword's LLR array in LDS.
+template <class CF, int LI>
+__device__ __forceinline__ void run_layer(uint16_t *Ls, uint32_t (&R)[CF::NR], int t, float beta_e, float beta3) {
+    constexpr ar4ja::Layer Ly = CF::S.layer[LI];
+    constexpr int D = Ly.deg, R0 = CF::r_off(LI), Z = CF::Z, G = CF::G;
+    const float beta = D == 3 ? beta3 : beta_e;
+    // Opaque copy of the thread index: keeps the shifted LDS addresses from
+    // being hoisted out of the iteration loop (they would pin VGPRs).
+    asm volatile("" : "+v"(t));
+    if constexpr (CF::C == 2) {
+        int ia[D], ib[D];
+        uint32_t qp[D];
+        float qa[D], qb[D], ma[D], mb[D];
+        uint32_t sp = 0;  // sign parity of both checks: bit 15 (t), bit 31 (t + G)
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            const int base = Ly.slot[d].base, sh = Ly.slot[d].shift;
+            const int xa = sh == 0 ? t : ((t + sh) & (Z - 1));
+            const int xb = sh == 0 ? t + G : (xa ^ G);  // (t + G + sh) mod Z, G = Z/2
+            ia[d] = base + xa;
+            ib[d] = base + xb;
+            const uint32_t rp = R[R0 + d];
+            qp[d] = cvt_pk_bf16(__fsub_rn(ld_bf(Ls + ia[d]), lo_f(rp)), __fsub_rn(ld_bf(Ls + ib[d]), hi_f(rp)));
+            qa[d] = lo_f(qp[d]);
+            qb[d] = hi_f(qp[d]);
+            ma[d] = fabsf(qa[d]);
+            mb[d] = fabsf(qb[d]);
+            sp ^= qp[d];
+        }
+        float ra[D], rb[D];
+        tree<D>(ma, beta, ra);
+        tree<D>(mb, beta, rb);
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            const uint32_t rn = cvt_pk_bf16(ra[d], rb[d]) | ((sp ^ qp[d]) & 0x80008000u);
+            const uint32_t ln = cvt_pk_bf16(__fadd_rn(qa[d], lo_f(rn)), __fadd_rn(qb[d], hi_f(rn)));
+            Ls[ia[d]] = (uint16_t)ln;
+            Ls[ib[d]] = (uint16_t)(ln >> 16);
+            R[R0 + d] = rn;
+        }
+    } else {
+        constexpr int DW = (D + 1) / 2;
+        int ia[2 * DW];
+        uint32_t qw[DW];
+        float q[2 * DW], m[D];
+        uint32_t sp = 0;  // bit 15: parity of even edges, bit 31: odd edges
+#pragma unroll
+        for (int w = 0; w < DW; ++w) {
+            const int d0 = 2 * w, d1 = 2 * w + 1;
+            const uint32_t rp = R[R0 + w];
+            ia[d0] = Ly.slot[d0].base + ((t + Ly.slot[d0].shift) & (Z - 1));
+            const float a = __fsub_rn(ld_bf(Ls + ia[d0]), lo_f(rp));
+            float b = 0.0f;
+            if (d1 < D) {
+                ia[d1] = Ly.slot[d1].base + ((t + Ly.slot[d1].shift) & (Z - 1));
+                b = __fsub_rn(ld_bf(Ls + ia[d1]), hi_f(rp));
+            }
+            qw[w] = cvt_pk_bf16(a, b);
+            q[d0] = lo_f(qw[w]);
+            q[d1] = hi_f(qw[w]);
+            sp ^= qw[w];
+        }
+#pragma unroll
+        for (int d = 0; d < D; ++d) m[d] = fabsf(q[d]);
+        const uint32_t s1 = (sp ^ (sp >> 16)) & 0x8000u, s2 = s1 | (s1 << 16);
+        float r[D];
+        tree<D>(m, beta, r);
+#pragma unroll
+        for (int w = 0; w < DW; ++w) {
+            const int d0 = 2 * w, d1 = 2 * w + 1;
+            const uint32_t rn = cvt_pk_bf16(r[d0], d1 < D ? r[d1 < D ? d1 : 0] : 0.0f) | ((s2 ^ qw[w]) & 0x80008000u);
+            const uint32_t ln = cvt_pk_bf16(__fadd_rn(q[d0], lo_f(rn)), __fadd_rn(q[d1], hi_f(rn)));
+            Ls[ia[d0]] = (uint16_t)ln;
+            if (d1 < D) Ls[ia[d1]] = (uint16_t)(ln >> 16);
+            R[R0 + w] = rn;
+        }
+    }
+}
+
+template <class CF, int... LI>
+__device__ __forceinline__ void run_iteration(uint16_t *Ls, uint32_t (&R)[CF::NR], int t, float beta_e, float beta3,
+                                              std::integer_sequence<int, LI...>) {
+    if constexpr (CF::WAVE)
+        ((run_layer<CF, LI>(Ls, R, t, beta_e, beta3), wave_fence()), ...);
+    else
+        ((run_layer<CF, LI>(Ls, R, t, beta_e, beta3), __syncthreads()), ...);
+}
+
+// 1 if any of the thread's checks of layer LI is unsatisfied.
+template <class CF, int LI>
+__device__ __forceinline__ bool layer_fails(const uint16_t *Ls, int t) {
+    constexpr ar4ja::Layer Ly = CF::S.layer[LI];
+    constexpr int D = Ly.deg, Z = CF::Z, G = CF::G;
+    asm volatile("" : "+v"(t));  // addresses stay inside the loop (see run_layer)
+    uint32_t pa = 0, pb = 0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        const int x = (t + Ly.slot[d].shift) & (Z - 1);
+        pa ^= hard_neg(Ls[Ly.slot[d].base + x]);
+        if constexpr (CF::C == 2) pb ^= hard_neg(Ls[Ly.slot[d].base + (x ^ G)]);
+    }
+    return (pa | pb) != 0u;
+}
+
+// Bit s of the result: some lane of wave slot s (lanes s*G .. s*G+G-1) is set.
+template <class CF>
+__device__ __forceinline__ uint32_t slot_any(uint64_t b) {
+    if constexpr (CF::SLOTS == 1) {
+        return b != 0 ? 1u : 0u;
+    } else {
+        uint32_t r = 0;
+#pragma unroll
+        for (int s = 0; s < CF::SLOTS; ++s)
+            if ((b >> (s * CF::G)) & ((1ull << CF::G) - 1ull)) r |= 1u << s;
+        return r;
+    }
+}
+
+// Wave mode: of the slots in `pass`, those whose every check is satisfied.
+// Layers are tested in order and the test stops once every slot has an
+// unsatisfied check, so an unconverged iteration usually costs one layer.
+template <class CF, int LI>
+__device__ __forceinline__ uint32_t syndrome_pass(const uint16_t *Ls, int t, uint32_t pass) {
+    if constexpr (LI == NLAYERS) {
+        return pass;
+    } else {
+        asm volatile("" ::: "memory");
+        pass &= ~slot_any<CF>(__ballot(layer_fails<CF, LI>(Ls, t)));
+        if (pass == 0) return 0;
+        return syndrome_pass<CF, LI + 1>(Ls, t, pass);
+    }
+}
+
+// Group mode: does any check of this wave's threads fail?
+template <class CF, int LI>
+__device__ __forceinline__ bool wave_fails_from(const uint16_t *Ls, int t) {
+    if constexpr (LI == NLAYERS) {
+        return false;
+    } else {
+        asm volatile("" ::: "memory");
+        if (__ballot(layer_fails<CF, LI>(Ls, t)) != 0) return true;
+        return wave_fails_from<CF, LI + 1>(Ls, t);
+    }
+}
+
+// Channel LLRs of codeword w by the G threads of its owner: Philox(ctr =
+// (w_lo, w_hi, blk, 0), key = seed) -> 4 normals -> RNE_bf16(mu + sigma z);
+// punctured variables = +0.
+template <class CF>
+__device__ __forceinline__ void gen_channel(uint16_t *Ls, int t, uint64_t seed, uint64_t w, float mu, float sigma) {
+    const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
+    for (int b = t; b < CF::NTX / 4; b += CF::G) {
+        const u32x4 x = philox4x32_10((uint32_t)w, (uint32_t)(w >> 32), (uint32_t)b, 0u, k0, k1);
+        float z0, z1, z2, z3;
+        box_muller(x.x, x.y, z0, z1);
+        box_muller(x.z, x.w, z2, z3);
+        const uint32_t lo = (llr_bits(mu, sigma, z1) & 0xFFFF0000u) | (llr_bits(mu, sigma, z0) >> 16);
+        const uint32_t hi = (llr_bits(mu, sigma, z3) & 0xFFFF0000u) | (llr_bits(mu, sigma, z2) >> 16);
+        *reinterpret_cast<uint2 *>(Ls + 4 * b) = make_uint2(lo, hi);
+    }
+    for (int b = t; b < CF::NP / 4; b += CF::G) *reinterpret_cast<uint2 *>(Ls + CF::NTX + 4 * b) = make_uint2(0u, 0u);
+}
+
+template <class CF>
+__device__ __forceinline__ void copy_in(uint16_t *Ls, const uint16_t *src, int t) {
+    for (int i = t; i < CF::N / 4; i += CF::G)
+        reinterpret_cast<uint2 *>(Ls)[i] = reinterpret_cast<const uint2 *>(src)[i];
+}
+
+template <class CF>
+__device__ __forceinline__ void copy_out(uint16_t *dst, const uint16_t *Ls, int t) {
+    for (int i = t; i < CF::N / 4; i += CF::G)
+        reinterpret_cast<uint2 *>(dst)[i] = reinterpret_cast<const uint2 *>(Ls)[i];
+}
+
+template <class CF>
+__device__ __forceinline__ uint32_t info_errors(const uint16_t *Ls, int t) {
+    uint32_t be = 0;
+    for (int i = t; i < CF::K; i += CF::G) be += nonpos(Ls[i]);
+    return be;
+}
+
+__device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+struct Run {
+    uint64_t seed, first_cw;
+    uint32_t n_cw;
+    float mu, sigma;
+};
+
+__device__ __forceinline__ Run load_run(const MultiParams &P) {
+    Run r{P.seed, P.first_cw, P.n_cw, P.mu, P.sigma};
+    if (P.dyn) {
+        r.seed = P.dyn->seed;
+        r.first_cw = P.dyn->first_cw;
+        r.n_cw = P.dyn->n_cw;
+        r.mu = P.dyn->mu;
+        r.sigma = P.dyn->sigma;
+    }
+    return r;
+}
+
+__device__ __forceinline__ void flush_counters(const MultiParams &P, unsigned long long blocks, unsigned long long fe,
+                                               unsigned long long be, unsigned long long its) {
+    if (blocks) atomicAdd(P.counters + 0, blocks);
+    if (fe) atomicAdd(P.counters + 1, fe);
+    if (be) atomicAdd(P.counters + 2, be);
+    if (its) atomicAdd(P.counters + 3, its);
+}
+
+__device__ __forceinline__ void hist_add(uint32_t *hist_s, const MultiParams &P, int bin) {
+    if (bin < HIST_LDS)
+        atomicAdd(hist_s + bin, 1u);
+    else
+        atomicAdd(P.hist + bin, 1ull);
+}
+
+__device__ __forceinline__ void hist_flush(const uint32_t *hist_s, const MultiParams &P, int threads) {
+    const int nb = P.max_iter + 1 < HIST_LDS ? P.max_iter + 1 : HIST_LDS;
+    for (int i = threadIdx.x; i < nb; i += threads)
+        if (hist_s[i]) atomicAdd(P.hist + i, (unsigned long long)hist_s[i]);
+}
+
+// ---------------------------------------------------------------------------
+// Wave mode
+// ---------------------------------------------------------------------------
+template <class CF>
+__global__ void __launch_bounds__(CF::THREADS) k_multi_wave(MultiParams P) {
+    __shared__ __attribute__((aligned(16))) uint16_t smem[CF::CW_PER_WG * CF::N];
+    __shared__ uint32_t hist_s[HIST_LDS];
+    const Run run = load_run(P);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int slot = lane / CF::G, t = lane % CF::G;
+    uint16_t *Ls = smem + (wv * CF::SLOTS + slot) * CF::N;
+    for (int i = threadIdx.x; i < HIST_LDS; i += CF::THREADS) hist_s[i] = 0u;
+    __syncthreads();
+
+    // Work split: the first 7/8 of the launch is dealt round-robin to the
+    // slots of the grid (no atomics), the rest comes from the queue.
+    const uint32_t n_slots = gridDim.x * CF::WPW * CF::SLOTS;
+    const uint32_t gslot = (blockIdx.x * CF::WPW + wv) * CF::SLOTS + slot;
+    const uint32_t n_static = (uint32_t)(((uint64_t)run.n_cw * 7u / 8u) / n_slots * n_slots);
+    const uint32_t all_slots = (1u << CF::SLOTS) - 1u;
+    uint32_t kstat = 0, cw = run.n_cw;
+    int it = 0;
+    uint32_t R[CF::NR];
+#pragma unroll
+    for (int i = 0; i < CF::NR; ++i) R[i] = 0u;
+    unsigned long long blocks = 0, fe = 0, be = 0, its = 0;
+    uint32_t done = all_slots, conv = 0, active = 0;
+    for (;;) {
+        if (done) {
+            const bool mine = (done >> slot) & 1u;
+            if (mine && cw < run.n_cw) {
+                const bool ok = (conv >> slot) & 1u;
+                if (!ok) be += info_errors<CF>(Ls, t);
+                if (P.llr_in) copy_out<CF>(P.llr_out + (size_t)cw * CF::N, Ls, t);
+                if (t == 0) {
+                    blocks += 1;
+                    its += ok ? (unsigned)it : (unsigned)P.max_iter;
+                    fe += ok ? 0u : 1u;
+                    hist_add(hist_s, P, ok ? it : 0);
+                    if (P.llr_in) {
+                        P.iters[cw] = ok ? (uint32_t)it : (uint32_t)P.max_iter;
+                        P.ok[cw] = ok ? 1 : 0;
+                    }
+                }
+            }
+            uint32_t nxt = run.n_cw;
+            if (mine && t == 0) {
+                const uint32_t s = gslot + kstat * n_slots;
+                if (s < n_static) {
+                    nxt = s;
+                } else {
+                    const uint32_t v = atomicAdd(P.queue, 1u);
+                    nxt = v >= run.n_cw - n_static ? run.n_cw : n_static + v;
+                }
+            }
+            nxt = __shfl(nxt, slot * CF::G);
+            if (mine) {
+                kstat += 1;
+                cw = nxt;
+                it = 0;
+                if (cw < run.n_cw) {
+                    if (P.llr_in)
+                        copy_in<CF>(Ls, P.llr_in + (size_t)cw * CF::N, t);
+                    else
+                        gen_channel<CF>(Ls, t, run.seed, run.first_cw + cw, run.mu, run.sigma);
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < CF::NR; ++i) R[i] = mine ? 0u : R[i];  // bf16 +0: iteration 0 has q = L
+            active = slot_any<CF>(__ballot(cw < run.n_cw));
+            wave_fence();
+        }
+        if (!active) break;
+        run_iteration<CF>(Ls, R, t, P.beta, P.beta3, std::make_integer_sequence<int, NLAYERS>{});
+        it += 1;
+        conv = syndrome_pass<CF, 0>(Ls, t, active);
+        done = (conv | slot_any<CF>(__ballot(it >= P.max_iter))) & active;
+    }
+    blocks = wave_sum(blocks);
+    fe = wave_sum(fe);
+    be = wave_sum(be);
+    its = wave_sum(its);
+    if (lane == 0) flush_counters(P, blocks, fe, be, its);
+    __syncthreads();
+    hist_flush(hist_s, P, CF::THREADS);
+}
+
+// ---------------------------------------------------------------------------
+// Group mode
+// ---------------------------------------------------------------------------
+template <class CF>
+__global__ void __launch_bounds__(CF::THREADS) k_multi_group(MultiParams P) {
+    __shared__ __attribute__((aligned(16))) uint16_t Ls[CF::N];
+    __shared__ uint32_t hist_s[HIST_LDS];
+    __shared__ uint32_t flag_s[CF::WPW];
+    __shared__ uint32_t cw_s, be_s;
+    const Run run = load_run(P);
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    for (int i = t; i < HIST_LDS; i += CF::THREADS) hist_s[i] = 0u;
+    const uint32_t n_wg = gridDim.x;
+    const uint32_t n_static = (uint32_t)(((uint64_t)run.n_cw * 7u / 8u) / n_wg * n_wg);
+    uint32_t kstat = 0;
+    unsigned long long blocks = 0, fe = 0, be = 0, its = 0;
+    uint32_t R[CF::NR];
+    __syncthreads();
+    for (;;) {
+        uint32_t cw;
+        const uint32_t s = blockIdx.x + kstat * n_wg;
+        if (s < n_static) {
+            cw = s;
+            ++kstat;
+        } else {
+            if (t == 0) {
+                const uint32_t v = atomicAdd(P.queue, 1u);
+                cw_s = v >= run.n_cw - n_static ? run.n_cw : n_static + v;
+            }
+            __syncthreads();
+            cw = cw_s;
+        }
+        if (cw >= run.n_cw) break;
+        if (P.llr_in)
+            copy_in<CF>(Ls, P.llr_in + (size_t)cw * CF::N, t);
+        else
+            gen_channel<CF>(Ls, t, run.seed, run.first_cw + cw, run.mu, run.sigma);
+        if (t == 0) be_s = 0u;
+#pragma unroll
+        for (int i = 0; i < CF::NR; ++i) R[i] = 0u;
+        __syncthreads();
+        int it = 0;
+        bool ok = false;
+        while (it < P.max_iter) {
+            run_iteration<CF>(Ls, R, t, P.beta, P.beta3, std::make_integer_sequence<int, NLAYERS>{});
+            ++it;
+            const bool f = wave_fails_from<CF, 0>(Ls, t);
+            if (lane == 0) flag_s[wv] = f ? 1u : 0u;
+            __syncthreads();
+            uint32_t any = 0;
+#pragma unroll
+            for (int w = 0; w < CF::WPW; ++w) any |= flag_s[w];
+            if (!any) {
+                ok = true;
+                break;
+            }
+        }
+        if (!ok) {
+            const uint32_t e = info_errors<CF>(Ls, t);
+            if (e) atomicAdd(&be_s, e);
+        }
+        if (P.llr_in) copy_out<CF>(P.llr_out + (size_t)cw * CF::N, Ls, t);
+        __syncthreads();
+        if (t == 0) {
+            blocks += 1;
+            its += ok ? (unsigned)it : (unsigned)P.max_iter;
+            if (!ok) {
+                fe += 1;
+                be += be_s;
+            }
+            hist_add(hist_s, P, ok ? it : 0);
+            if (P.llr_in) {
+                P.iters[cw] = ok ? (uint32_t)it : (uint32_t)P.max_iter;
+                P.ok[cw] = ok ? 1 : 0;
+            }
+        }
+        __syncthreads();
+    }
+    if (t == 0) flush_counters(P, blocks, fe, be, its);
+    __syncthreads();
+    hist_flush(hist_s, P, CF::THREADS);
+}
+
+template <class CF>
+constexpr auto kernel_of() {
+    if constexpr (CF::WAVE)
+        return &k_multi_wave<CF>;
+    else
+        return &k_multi_group<CF>;
+}
+
+template <class CF>
+hipError_t launch(const MultiParams &p, int grid, hipStream_t st) {
+    hipLaunchKernelGGL(kernel_of<CF>(), dim3(grid), dim3(CF::THREADS), 0, st, p);
+    return hipGetLastError();
+}
+
+template <class CF>
+hipError_t occupancy(int *wg_per_cu) {
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(wg_per_cu, kernel_of<CF>(), CF::THREADS, 0);
+}
+
+template <class CF>
+hipError_t attributes(hipFuncAttributes *a) {
+    return hipFuncGetAttributes(a, reinterpret_cast<const void *>(kernel_of<CF>()));
+}
+
+template <class CF>
+constexpr MultiVariant variant() {
+    return MultiVariant{LDPC_KID, LDPC_RID, CF::C, CF::ORD, CF::THREADS, CF::CW_PER_WG,
+                        &launch<CF>, &occupancy<CF>, &attributes<CF>};
+}
+
+// C = 1 needs Z <= 1024 threads per codeword (all codes but K = 16384 r1/2).
+constexpr bool C1_OK = ar4ja::make_schedule(LDPC_KID, LDPC_RID).Z <= 1024;
+
+const MultiVariant TABLE[] = {
+    variant<Cfg<2, 0>>(),
+    variant<Cfg<2, 1>>(),
+#if !(LDPC_KID == 2 && LDPC_RID == 0)
+    variant<Cfg<1, 0>>(),
+    variant<Cfg<1, 1>>(),
+#endif
+};
+static_assert(C1_OK == !(LDPC_KID == 2 && LDPC_RID == 0), "C = 1 availability");
+
+}  // namespace
+
+#define LDPC_CAT3(a, b, c, d) a##b##c##d
+#define LDPC_TABLE_NAME(k, r) LDPC_CAT3(multi_table_k, k, _r, r)
+
+const MultiVariant *LDPC_TABLE_NAME(LDPC_KID, LDPC_RID)(int *n) {
+    *n = (int)(sizeof(TABLE) / sizeof(TABLE[0]));
+    return TABLE;
+}
+
+}  // namespace ldpc

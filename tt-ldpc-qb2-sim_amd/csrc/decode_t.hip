@@ -86,19 +86,6 @@ __device__ __forceinline__ void check_tree(const float (&q)[D], float beta, floa
     }
 }
 
-// Two fp32 values -> packed bf16 pair (lo = a, hi = b), round to nearest
-// even.  v_cvt_pk_bf16_f32 was checked exhaustively against the integer
-// formula (u + 0x7FFF + lsb) >> 16 on every non-NaN fp32 pattern (2^32 - 2^24
-// + 2 inputs, 0 mismatches, MI355X); NaN never reaches it here.
-typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
-typedef float f32x2_t __attribute__((ext_vector_type(2)));
-__device__ __forceinline__ uint32_t cvt_pk_bf16(float a, float b) {
-    const bf16x2_t r = __builtin_convertvector((f32x2_t){a, b}, bf16x2_t);
-    return __builtin_bit_cast(uint32_t, r);
-}
-__device__ __forceinline__ float lo_f(uint32_t p) { return u2f(p << 16); }
-__device__ __forceinline__ float hi_f(uint32_t p) { return u2f(p & 0xFFFF0000u); }
-
 // One layer: lane updates checks j = lane (a) and j = lane + 64 (b).  q, R'
 // and L' of the two checks travel as packed bf16 pairs; the signs of R' are
 // applied to the packed word (RNE commutes with the sign bit).

@@ -38,6 +38,46 @@ hipError_t r_launch_gen(uint64_t seed, uint64_t first_cw, uint32_t n_cw, float m
 hipError_t r_launch_mc(const RLaunch &L, uint64_t seed, uint64_t first_cw, uint32_t n_cw, float mu, float sigma,
                        unsigned long long *counters, unsigned long long *hist, hipStream_t st);
 
+// Profile T for the nine AR4JA codes (decode_multi.hip, one object per code).
+// Per-launch values a captured graph changes between replays, read by the
+// kernel from device memory when MultiParams::dyn is set.
+struct MultiDyn {
+    uint64_t seed, first_cw;
+    uint32_t n_cw;
+    float mu, sigma;
+    uint32_t reserved;
+};
+struct MultiParams {
+    const uint16_t *llr_in;  // decode mode: n_cw x N bf16 input; null for Monte Carlo
+    uint16_t *llr_out;       // decode mode output
+    uint32_t *iters;
+    uint8_t *ok;
+    uint32_t *queue;               // one u32, zero at launch
+    unsigned long long *counters;  // blocks, frame errors, bit errors, iteration sum
+    unsigned long long *hist;      // max_iter + 1 bins
+    const MultiDyn *dyn;
+    uint64_t seed, first_cw;
+    uint32_t n_cw;
+    int max_iter;
+    float mu, sigma, beta, beta3;
+};
+struct MultiVariant {
+    int kid, rid, c, order;  // code (K = 1024 << 2 kid, rate 1/2, 2/3, 4/5), checks per thread, layer order
+    int threads, cw_per_wg;  // workgroup size, codewords in flight per workgroup
+    hipError_t (*launch)(const MultiParams &p, int grid, hipStream_t st);
+    hipError_t (*occupancy)(int *wg_per_cu);
+    hipError_t (*attributes)(hipFuncAttributes *a);
+};
+const MultiVariant *multi_table_k0_r0(int *n);
+const MultiVariant *multi_table_k0_r1(int *n);
+const MultiVariant *multi_table_k0_r2(int *n);
+const MultiVariant *multi_table_k1_r0(int *n);
+const MultiVariant *multi_table_k1_r1(int *n);
+const MultiVariant *multi_table_k1_r2(int *n);
+const MultiVariant *multi_table_k2_r0(int *n);
+const MultiVariant *multi_table_k2_r1(int *n);
+const MultiVariant *multi_table_k2_r2(int *n);
+
 hipError_t launch_fill_u64(unsigned long long *p, size_t n, unsigned long long v, hipStream_t st);
 hipError_t launch_fill_u32(uint32_t *p, size_t n, uint32_t v, hipStream_t st);
 

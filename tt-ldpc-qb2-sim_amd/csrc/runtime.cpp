@@ -14,6 +14,7 @@
 #include "../../include/ldpc_mi355x.h"
 #include "ar4ja_schedule.h"
 #include "kernels.h"
+#include "runtime_internal.h"
 
 namespace {
 
@@ -37,6 +38,9 @@ struct ldpc_ctx {
     uint32_t *d_queue = nullptr;
     unsigned long long *d_counters = nullptr;  // [4]
     unsigned long long *d_hist = nullptr;      // [max_iter + 1]
+    // Profile T on the generic nine-code kernels (null: the specialised
+    // AR4JA r1/2 K=1024 kernel)
+    const ldpc::MultiVariant *mv = nullptr;
     // Profile R: level-ordered pass table on the device
     void *d_table = nullptr;
     ldpc::RLaunch rl{};
@@ -73,15 +77,6 @@ int ensure(ldpc_ctx *c, DevBuf &b, size_t bytes) {
 
 bool is_r12_k1024(const ldpc_code_desc &d) {
     return d.family == LDPC_CODE_AR4JA && d.k_info == 1024 && d.rate_num == 1 && d.rate_den == 2;
-}
-
-bool ar4ja_ids(const ldpc_code_desc &d, int &k_id, int &rate_id) {
-    k_id = d.k_info == 1024 ? 0 : d.k_info == 4096 ? 1 : d.k_info == 16384 ? 2 : -1;
-    rate_id = (d.rate_num == 1 && d.rate_den == 2)   ? 0
-              : (d.rate_num == 2 && d.rate_den == 3) ? 1
-              : (d.rate_num == 4 && d.rate_den == 5) ? 2
-                                                     : -1;
-    return k_id >= 0 && rate_id >= 0;
 }
 
 // Parity-check rows of an AR4JA code in the natural (makeH) row order, each
@@ -203,6 +198,67 @@ int setup_profile_r(ldpc_ctx *c, const std::vector<int32_t> &rp, const std::vect
 
 }  // namespace
 
+namespace ldpc {
+
+bool ar4ja_ids(const ldpc_code_desc &d, int &k_id, int &rate_id) {
+    k_id = d.family != LDPC_CODE_AR4JA ? -1
+           : d.k_info == 1024          ? 0
+           : d.k_info == 4096          ? 1
+           : d.k_info == 16384         ? 2
+                                       : -1;
+    rate_id = (d.rate_num == 1 && d.rate_den == 2)   ? 0
+              : (d.rate_num == 2 && d.rate_den == 3) ? 1
+              : (d.rate_num == 4 && d.rate_den == 5) ? 2
+                                                     : -1;
+    return k_id >= 0 && rate_id >= 0;
+}
+
+const MultiVariant *multi_table(int k_id, int rate_id, int *n) {
+    static const MultiVariant *(*const T[3][3])(int *) = {
+        {multi_table_k0_r0, multi_table_k0_r1, multi_table_k0_r2},
+        {multi_table_k1_r0, multi_table_k1_r1, multi_table_k1_r2},
+        {multi_table_k2_r0, multi_table_k2_r1, multi_table_k2_r2}};
+    return T[k_id][rate_id](n);
+}
+
+uint32_t variant_code(const MultiVariant &v) { return LDPC_VARIANT(v.c, v.order); }
+
+const MultiVariant *find_variant(int k_id, int rate_id, uint32_t variant) {
+    if (!(variant & 0x100u) || (variant & ~0x1FFu)) return nullptr;
+    int n = 0;
+    const MultiVariant *t = multi_table(k_id, rate_id, &n);
+    for (int i = 0; i < n; ++i)
+        if (variant_code(t[i]) == variant) return &t[i];
+    return nullptr;
+}
+
+// Default variant per code (layer order 0): checks per thread from
+// ldpc_autotune runs on an MI355X at each code's waterfall Eb/N0.
+uint32_t default_variant(int k_id, int rate_id) {
+    static const int C[3][3] = {{2, 1, 1}, {2, 1, 1}, {2, 1, 1}};
+    return LDPC_VARIANT(C[k_id][rate_id], 0);
+}
+
+void fill_code_info(int k_id, int rate_id, ldpc_code_info &info) {
+    const ar4ja::Schedule s = ar4ja::make_schedule(k_id, rate_id);
+    info.n = (uint32_t)s.N;
+    info.m = (uint32_t)s.n_checks;
+    info.k_info = (uint32_t)s.k_info;
+    info.n_tx = (uint32_t)(s.N - s.n_punct);
+    info.z = (uint32_t)s.Z;
+    info.n_layers = (uint32_t)s.n_layers;
+    info.n_edges = (uint32_t)s.n_edges;
+}
+
+void channel_params(const ldpc_code_info &info, double ebn0_db, float *mu, float *sigma) {
+    const double rate = (double)info.k_info / (double)info.n_tx;
+    const double g = std::pow(10.0, ebn0_db / 10.0);
+    *mu = (float)(4.0 * rate * g);
+    *sigma = (float)std::sqrt(8.0 * rate * g);
+}
+
+}  // namespace ldpc
+
 extern "C" {
 
 const char *ldpc_last_error(const ldpc_ctx *ctx) { return ctx ? ctx->err.c_str() : g_create_error.c_str(); }
@@ -219,7 +275,7 @@ int ldpc_create(const ldpc_code_desc *code, const ldpc_decoder_cfg *cfg, int dev
     if (code->family != LDPC_CODE_AR4JA && code->family != LDPC_CODE_MATRIX)
         return fail(nullptr, LDPC_ERR_ARG, "unknown code family");
     int k_id = -1, rate_id = -1;
-    if (code->family == LDPC_CODE_AR4JA && !ar4ja_ids(*code, k_id, rate_id))
+    if (code->family == LDPC_CODE_AR4JA && !ldpc::ar4ja_ids(*code, k_id, rate_id))
         return fail(nullptr, LDPC_ERR_ARG, "no AR4JA code with this (k_info, rate)");
     if (cfg->profile > LDPC_PROFILE_R_NMS) return fail(nullptr, LDPC_ERR_UNSUPPORTED, "unknown decoder profile");
     if (cfg->max_iter < 1 || cfg->max_iter > 100000) return fail(nullptr, LDPC_ERR_ARG, "max_iter out of range");
@@ -228,9 +284,12 @@ int ldpc_create(const ldpc_code_desc *code, const ldpc_decoder_cfg *cfg, int dev
         if (!(cfg->beta > 0.0f) || !std::isfinite(cfg->beta)) return fail(nullptr, LDPC_ERR_ARG, "beta must be > 0");
         if (cfg->l_max != 0.0f || cfg->r_max != 0.0f || cfg->l_post_max != 0.0f || cfg->gamma != 0.0f)
             return fail(nullptr, LDPC_ERR_ARG, "clamps belong to Profile R; the golden Profile T has none");
-        if (!is_r12_k1024(*code))
-            return fail(nullptr, LDPC_ERR_UNSUPPORTED, "Profile T kernels are built for AR4JA r1/2 K=1024 only");
+        if (code->family != LDPC_CODE_AR4JA)
+            return fail(nullptr, LDPC_ERR_UNSUPPORTED, "Profile T kernels are built for the AR4JA codes");
+        if (cfg->variant && !ldpc::find_variant(k_id, rate_id, cfg->variant))
+            return fail(nullptr, LDPC_ERR_UNSUPPORTED, "no such Profile T kernel variant for this code");
     } else {
+        if (cfg->variant) return fail(nullptr, LDPC_ERR_ARG, "kernel variants belong to Profile T");
         for (float v : {cfg->l_max, cfg->r_max, cfg->l_post_max, cfg->gamma})
             if (!(v >= 0.0f) || !std::isfinite(v)) return fail(nullptr, LDPC_ERR_ARG, "clamps must be >= 0");
         if (code->family == LDPC_CODE_MATRIX &&
@@ -268,14 +327,14 @@ int ldpc_create(const ldpc_code_desc *code, const ldpc_decoder_cfg *cfg, int dev
     if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device)) != hipSuccess)
         return bail(e, "hipDeviceGetAttribute");
 
-    if (prof_t) {
-        c->info.n = 2560;
-        c->info.m = 1536;
-        c->info.k_info = 1024;
-        c->info.n_tx = 2048;
-        c->info.z = 128;
-        c->info.n_layers = 12;
-        c->info.n_edges = 7680;
+    if (prof_t && !(is_r12_k1024(*code) && cfg->variant == 0)) {
+        c->mv = ldpc::find_variant(k_id, rate_id, cfg->variant ? cfg->variant : ldpc::default_variant(k_id, rate_id));
+        ldpc::fill_code_info(k_id, rate_id, c->info);
+        int occ = 0;
+        if ((e = c->mv->occupancy(&occ)) != hipSuccess) return bail(e, "occupancy");
+        c->grid_mc = c->grid_dec = cus * (occ > 0 ? occ : 1);
+    } else if (prof_t) {
+        ldpc::fill_code_info(0, 0, c->info);
         int occ_mc = 0, occ_dec = 0;
         if ((e = ldpc::t12_occupancy(&occ_mc)) != hipSuccess) return bail(e, "occupancy");
         if ((e = ldpc::t12_occupancy_decode(&occ_dec)) != hipSuccess) return bail(e, "occupancy");
@@ -341,19 +400,43 @@ int ldpc_get_code_info(const ldpc_ctx *c, ldpc_code_info *info) {
     return LDPC_OK;
 }
 
-int ldpc_get_schedule(const ldpc_code_desc *code, int32_t *out, int32_t cap) {
-    if (!code || !out || cap <= 0) return LDPC_ERR_ARG;
-    if (!is_r12_k1024(*code)) return LDPC_ERR_UNSUPPORTED;
-    int n = ldpc::t12_schedule(out, cap);
-    return n < 0 ? LDPC_ERR_ARG : n;
+int ldpc_get_schedule(const ldpc_code_desc *code, uint32_t layer_order, int32_t *out, int32_t cap) {
+    if (!code || !out || cap <= 0 || layer_order > 1) return LDPC_ERR_ARG;
+    int k_id, rate_id;
+    if (!ldpc::ar4ja_ids(*code, k_id, rate_id)) return LDPC_ERR_UNSUPPORTED;
+    if (k_id == 0 && rate_id == 0 && layer_order == 0) {
+        // the specialised kernel's own compiled table
+        const int n = ldpc::t12_schedule(out, cap);
+        return n < 0 ? LDPC_ERR_ARG : n;
+    }
+    const ar4ja::Schedule s = ar4ja::make_schedule(k_id, rate_id, (int)layer_order);
+    int n = 0;
+    for (int li = 0; li < s.n_layers; ++li) {
+        const ar4ja::Layer &L = s.layer[li];
+        if (n + 2 + 2 * L.deg > cap) return LDPC_ERR_ARG;
+        out[n++] = L.first_row;
+        out[n++] = L.deg;
+        for (int d = 0; d < L.deg; ++d) {
+            out[n++] = L.slot[d].base;
+            out[n++] = L.slot[d].shift;
+        }
+    }
+    return n;
+}
+
+int ldpc_list_variants(const ldpc_code_desc *code, uint32_t *out, int32_t cap) {
+    if (!code || (!out && cap > 0) || cap < 0) return LDPC_ERR_ARG;
+    int k_id, rate_id;
+    if (!ldpc::ar4ja_ids(*code, k_id, rate_id)) return LDPC_ERR_UNSUPPORTED;
+    int n = 0;
+    const ldpc::MultiVariant *t = ldpc::multi_table(k_id, rate_id, &n);
+    for (int i = 0; i < n && i < cap; ++i) out[i] = ldpc::variant_code(t[i]);
+    return n;
 }
 
 int ldpc_channel_params(const ldpc_ctx *c, double ebn0_db, float *mu, float *sigma) {
     if (!c || !mu || !sigma || !std::isfinite(ebn0_db)) return LDPC_ERR_ARG;
-    const double rate = (double)c->info.k_info / (double)c->info.n_tx;
-    const double g = std::pow(10.0, ebn0_db / 10.0);
-    *mu = (float)(4.0 * rate * g);
-    *sigma = (float)std::sqrt(8.0 * rate * g);
+    ldpc::channel_params(c->info, ebn0_db, mu, sigma);
     return LDPC_OK;
 }
 
@@ -367,6 +450,17 @@ int ldpc_sync(ldpc_ctx *c) {
 void *ldpc_stream(ldpc_ctx *c) { return c ? (void *)c->stream : nullptr; }
 
 static float l_max_at(const ldpc_ctx *c, float mu) { return c->cfg.gamma > 0.0f ? c->cfg.gamma * mu : c->cfg.l_max; }
+
+static ldpc::MultiParams multi_params(const ldpc_ctx *c) {
+    ldpc::MultiParams p{};
+    p.queue = c->d_queue;
+    p.counters = c->d_counters;
+    p.hist = c->d_hist;
+    p.max_iter = (int)c->cfg.max_iter;
+    p.beta = c->cfg.beta;
+    p.beta3 = c->cfg.beta < 1.0f ? c->cfg.beta : 1.0f;  // stepper:300-304
+    return p;
+}
 
 int ldpc_run_point(ldpc_ctx *c, const ldpc_point_cfg *pt, ldpc_point_result *res, uint64_t *hist) {
     if (!c || !pt || !res) return fail(c, LDPC_ERR_ARG, "null argument");
@@ -392,7 +486,15 @@ int ldpc_run_point(ldpc_ctx *c, const ldpc_point_cfg *pt, ldpc_point_result *res
         HIP_TRY(c, ldpc::launch_fill_u64(c->d_counters, 4, 0ull, c->stream));
         HIP_TRY(c, ldpc::launch_fill_u64(c->d_hist, nh, 0ull, c->stream));
         HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
-        if (prof_t)
+        if (c->mv) {
+            ldpc::MultiParams p = multi_params(c);
+            p.seed = pt->seed;
+            p.first_cw = next;
+            p.n_cw = (uint32_t)n;
+            p.mu = mu;
+            p.sigma = sigma;
+            HIP_TRY(c, c->mv->launch(p, c->grid_mc, c->stream));
+        } else if (prof_t)
             HIP_TRY(c, ldpc::t12_launch_mc(pt->seed, next, (uint32_t)n, mu, sigma, (int)c->cfg.max_iter, c->cfg.beta,
                                            c->d_queue, c->d_counters, c->d_hist, c->grid_mc, c->stream));
         else
@@ -432,7 +534,17 @@ int ldpc_decode_llrs_device_async(ldpc_ctx *c, const void *d_llr, uint32_t n_cw,
     if (!d_llr || !d_out_llr || !d_out_iters || !d_out_ok) return fail(c, LDPC_ERR_ARG, "null buffer");
     HIP_TRY(c, hipSetDevice(c->device));
     HIP_TRY(c, ldpc::launch_fill_u32(c->d_queue, 1, 0u, c->stream));
-    if (c->cfg.profile == LDPC_PROFILE_T)
+    if (c->mv) {
+        HIP_TRY(c, ldpc::launch_fill_u64(c->d_counters, 4, 0ull, c->stream));
+        HIP_TRY(c, ldpc::launch_fill_u64(c->d_hist, c->cfg.max_iter + 1, 0ull, c->stream));
+        ldpc::MultiParams p = multi_params(c);
+        p.llr_in = (const uint16_t *)d_llr;
+        p.llr_out = (uint16_t *)d_out_llr;
+        p.iters = (uint32_t *)d_out_iters;
+        p.ok = (uint8_t *)d_out_ok;
+        p.n_cw = n_cw;
+        HIP_TRY(c, c->mv->launch(p, c->grid_dec, c->stream));
+    } else if (c->cfg.profile == LDPC_PROFILE_T)
         HIP_TRY(c, ldpc::t12_launch_decode((const uint16_t *)d_llr, (uint16_t *)d_out_llr, (uint32_t *)d_out_iters,
                                            (uint8_t *)d_out_ok, n_cw, (int)c->cfg.max_iter, c->cfg.beta, c->d_queue,
                                            c->grid_dec, c->stream));
@@ -472,7 +584,10 @@ int ldpc_gen_channel(ldpc_ctx *c, uint64_t seed, uint64_t first_cw, uint32_t n_c
     int rc;
     if ((rc = ensure(c, c->out, nb))) return rc;
     HIP_TRY(c, hipSetDevice(c->device));
-    if (c->cfg.profile == LDPC_PROFILE_T)
+    if (c->mv)  // same Philox / Box-Muller stream, size-generic generator, no clamp
+        HIP_TRY(c, ldpc::r_launch_gen(seed, first_cw, n_cw, mu, sigma, 0.0f, (int)c->info.n_tx, (int)c->info.n,
+                                      (uint16_t *)c->out.p, c->stream));
+    else if (c->cfg.profile == LDPC_PROFILE_T)
         HIP_TRY(c, ldpc::t12_launch_gen(seed, first_cw, n_cw, mu, sigma, (uint16_t *)c->out.p, c->stream));
     else
         HIP_TRY(c, ldpc::r_launch_gen(seed, first_cw, n_cw, mu, sigma, l_max_at(c, mu), c->rl.n_tx, c->rl.n,

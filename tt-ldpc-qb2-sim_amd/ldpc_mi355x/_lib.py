@@ -32,7 +32,7 @@ class CodeDesc(C.Structure):
 
 class DecoderCfg(C.Structure):
     _fields_ = [("profile", C.c_uint32), ("max_iter", C.c_uint32), ("beta", C.c_float), ("l_max", C.c_float),
-                ("r_max", C.c_float), ("l_post_max", C.c_float), ("gamma", C.c_float), ("reserved", C.c_uint32)]
+                ("r_max", C.c_float), ("l_post_max", C.c_float), ("gamma", C.c_float), ("variant", C.c_uint32)]
 
 
 class CodeInfo(C.Structure):
@@ -49,13 +49,39 @@ class PointResult(C.Structure):
                                           "kernel_ns", "launches")]
 
 
+class TuneResult(C.Structure):
+    _fields_ = [("variant", C.c_uint32), ("threads_per_wg", C.c_uint32), ("cw_per_wg", C.c_uint32),
+                ("wg_per_cu", C.c_uint32), ("gsps", C.c_double), ("mean_iterations", C.c_double),
+                ("frame_errors", C.c_uint64), ("kernel_ns", C.c_uint64)]
+
+
+def variant(checks_per_thread: int, layer_order: int) -> int:
+    """LDPC_VARIANT(checks_per_thread, layer_order) of the header."""
+    return 0x100 | (checks_per_thread & 0xF) | ((layer_order & 0xF) << 4)
+
+
+def variant_fields(v: int):
+    """(checks_per_thread, layer_order) of a variant code."""
+    return v & 0xF, (v >> 4) & 0xF
+
+
 # name -> (restype, argtypes); every symbol include/ldpc_mi355x.h declares.
 SIGNATURES = {
     "ldpc_create": (C.c_int, [C.POINTER(CodeDesc), C.POINTER(DecoderCfg), C.c_int, C.POINTER(C.c_void_p)]),
     "ldpc_destroy": (None, [C.c_void_p]),
     "ldpc_last_error": (C.c_char_p, [C.c_void_p]),
     "ldpc_get_code_info": (C.c_int, [C.c_void_p, C.POINTER(CodeInfo)]),
-    "ldpc_get_schedule": (C.c_int, [C.POINTER(CodeDesc), C.c_void_p, C.c_int32]),
+    "ldpc_get_schedule": (C.c_int, [C.POINTER(CodeDesc), C.c_uint32, C.c_void_p, C.c_int32]),
+    "ldpc_list_variants": (C.c_int, [C.POINTER(CodeDesc), C.c_void_p, C.c_int32]),
+    "ldpc_autotune": (C.c_int, [C.POINTER(CodeDesc), C.POINTER(DecoderCfg), C.c_int, C.c_double, C.c_uint64,
+                                C.c_uint32, C.c_uint32, C.c_int, C.POINTER(TuneResult), C.c_int32,
+                                C.POINTER(C.c_uint32)]),
+    "ldpc_batch_create": (C.c_int, [C.POINTER(CodeDesc), C.c_void_p, C.c_uint32, C.POINTER(DecoderCfg), C.c_int,
+                                    C.POINTER(C.c_void_p)]),
+    "ldpc_batch_run": (C.c_int, [C.c_void_p, C.POINTER(PointCfg), C.POINTER(PointResult), C.c_void_p]),
+    "ldpc_batch_code_info": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(CodeInfo)]),
+    "ldpc_batch_last_error": (C.c_char_p, [C.c_void_p]),
+    "ldpc_batch_destroy": (None, [C.c_void_p]),
     "ldpc_row_levels": (C.c_int, [C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]),
     "ldpc_run_point": (C.c_int, [C.c_void_p, C.POINTER(PointCfg), C.POINTER(PointResult), C.c_void_p]),
     "ldpc_decode_llrs": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]),

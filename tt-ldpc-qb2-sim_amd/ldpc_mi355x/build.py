@@ -12,6 +12,7 @@ import os
 import shutil
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 PKG = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(PKG)
@@ -19,10 +20,14 @@ CSRC = os.path.join(ROOT, "csrc")
 LIB = os.path.join(PKG, "libldpc.so")
 ARCH = os.environ.get("LDPC_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["decode_t.hip", "decode_r.hip", "runtime.cpp"]
+SOURCES = ["decode_t.hip", "decode_r.hip", "runtime.cpp", "batch.cpp"]
+# decode_multi.hip is compiled once per AR4JA code (K id, rate id)
+MULTI = "decode_multi.hip"
+MULTI_CODES = [(k, r) for k in range(3) for r in range(3)]
 # command-line shims speaking the reference engines' CLI, linked to libldpc
 CLIS = {"ldpc_sim_scalar": "cli_scalar.cpp", "ldpc_sim_circulant": "cli_circulant.cpp"}
-HEADERS = ["ar4ja_schedule.h", "device_math.h", "kernels.h"]
+HEADERS = ["ar4ja_schedule.h", "device_math.h", "kernels.h", "runtime_internal.h"]
+OBJ_DIR = os.path.join(PKG, "_build")
 
 HIPCC_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-gpu-rdc",
@@ -38,15 +43,37 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found: the HIP toolchain is required to build libldpc")
 
 
-def build(force: bool = True, verbose: bool = False) -> str:
-    """Compile every HIP/C++ source into libldpc.so; returns its path."""
-    srcs = [os.path.join(CSRC, s) for s in SOURCES]
+def _compile_jobs():
+    jobs = [(os.path.join(CSRC, s), os.path.join(OBJ_DIR, s + ".o"), []) for s in SOURCES]
+    for k, r in MULTI_CODES:
+        jobs.append((os.path.join(CSRC, MULTI), os.path.join(OBJ_DIR, f"multi_k{k}_r{r}.o"),
+                     [f"-DLDPC_KID={k}", f"-DLDPC_RID={r}"]))
+    return jobs
+
+
+def build(force: bool = True, verbose: bool = False, jobs: int = 0) -> str:
+    """Compile every HIP/C++ source (in parallel) and link libldpc.so; returns its path."""
+    srcs = [os.path.join(CSRC, s) for s in SOURCES + [MULTI]]
     deps = srcs + [os.path.join(CSRC, h) for h in HEADERS] + [os.path.join(os.path.dirname(ROOT), "include",
                                                                            "ldpc_mi355x.h")]
     if not force and os.path.exists(LIB) and os.path.getmtime(LIB) >= max(os.path.getmtime(d) for d in deps):
         return LIB
+    os.makedirs(OBJ_DIR, exist_ok=True)
+    cc = hipcc()
+
+    def compile_one(job):
+        src, obj, defs = job
+        cmd = [cc, f"--offload-arch={ARCH}", *HIPCC_FLAGS, *defs, "-c", src, "-o", obj]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True, cwd=CSRC)
+        return obj
+
+    workers = jobs or min(16, os.cpu_count() or 4)
+    with ThreadPoolExecutor(max_workers=workers) as ex:
+        objs = list(ex.map(compile_one, _compile_jobs()))
     tmp = LIB + ".tmp"
-    cmd = [hipcc(), f"--offload-arch={ARCH}", *HIPCC_FLAGS, "-shared", "-o", tmp, *srcs]
+    cmd = [cc, f"--offload-arch={ARCH}", "-fPIC", "-shared", "-o", tmp, *objs]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True, cwd=CSRC)

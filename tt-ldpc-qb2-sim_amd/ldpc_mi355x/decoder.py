@@ -70,13 +70,20 @@ def row_levels(rows, n):
     return out
 
 
-def compiled_schedule(k_info: int = 1024, rate: str = "1/2"):
+def ar4ja_desc(k_info: int, rate: str) -> L.CodeDesc:
+    num, den = RATE_FRACTIONS[rate]
+    return L.CodeDesc(L.LDPC_CODE_AR4JA, k_info, num, den)
+
+
+AR4JA_CODES = [(k, r) for k in (1024, 4096, 16384) for r in ("1/2", "2/3", "4/5")]
+
+
+def compiled_schedule(k_info: int = 1024, rate: str = "1/2", layer_order: int = 0):
     """Layer schedule compiled into the kernels: [(first_row, [(base, shift), ...]), ...]."""
     lib = L.load()
-    num, den = RATE_FRACTIONS[rate]
-    desc = L.CodeDesc(L.LDPC_CODE_AR4JA, k_info, num, den)
+    desc = ar4ja_desc(k_info, rate)
     buf = np.zeros(8192, dtype=np.int32)
-    n = lib.ldpc_get_schedule(C.byref(desc), buf.ctypes.data, buf.size)
+    n = lib.ldpc_get_schedule(C.byref(desc), layer_order, buf.ctypes.data, buf.size)
     if n < 0:
         raise L.LdpcError(n, "no compiled schedule for this code")
     out, i = [], 0
@@ -87,14 +94,51 @@ def compiled_schedule(k_info: int = 1024, rate: str = "1/2"):
     return out
 
 
+def list_variants(k_info: int, rate: str):
+    """Profile T kernel variants built for an AR4JA code (LDPC_VARIANT codes)."""
+    lib = L.load()
+    buf = np.zeros(16, dtype=np.uint32)
+    n = lib.ldpc_list_variants(C.byref(ar4ja_desc(k_info, rate)), buf.ctypes.data, buf.size)
+    if n < 0:
+        raise L.LdpcError(n, "not an AR4JA code")
+    return [int(v) for v in buf[:n]]
+
+
 PROFILES = {"T": L.LDPC_PROFILE_T, "R": L.LDPC_PROFILE_R, "R_NMS": L.LDPC_PROFILE_R_NMS}
+
+
+def autotune(k_info: int, rate: str, ebn0_db: float, n_cw: int = 1 << 16, reps: int = 2, max_iter: int = 200,
+             beta: float = 0.9375, seed: int = 0x5EED2026, any_order: bool = False, device: int = 0):
+    """Time every Profile T kernel variant of an AR4JA code on the same codewords.
+
+    Returns (best variant, [dict per variant]); libldpc checks that variants
+    with the same layer order produce identical counters.
+    """
+    lib = L.load()
+    cfg = L.DecoderCfg(L.LDPC_PROFILE_T, max_iter, beta)
+    out = (L.TuneResult * 16)()
+    best = C.c_uint32()
+    n = lib.ldpc_autotune(C.byref(ar4ja_desc(k_info, rate)), C.byref(cfg), device, ebn0_db, seed, n_cw, reps,
+                          int(any_order), out, 16, C.byref(best))
+    if n < 0:
+        raise L.LdpcError(n, lib.ldpc_last_error(None).decode())
+    rows = []
+    for t in out[:n]:
+        c, order = L.variant_fields(t.variant)
+        rows.append(dict(variant=t.variant, checks_per_thread=c, layer_order=order, threads_per_wg=t.threads_per_wg,
+                         cw_per_wg=t.cw_per_wg, wg_per_cu=t.wg_per_cu, gsps=t.gsps,
+                         mean_iterations=t.mean_iterations, frame_errors=t.frame_errors,
+                         kernel_ms=t.kernel_ns * 1e-6))
+    return best.value, rows
 
 
 class Engine:
     """One libldpc context: a code, a decoder configuration and a device.
 
     Profiles:
-      "T"     the golden stepper's tree Approx-Min* (AR4JA r1/2 K=1024);
+      "T"     the golden stepper's tree Approx-Min* (the nine AR4JA codes;
+              ``variant`` = L.variant(checks_per_thread, layer_order) picks a
+              kernel variant, 0 the library default);
       "R"     the scalar_5w recursive Approx-Min* that run_simulation.py drives;
       "R_NMS" the same engine's normalized min-sum (its -n flag).
     A Profile R engine takes either an AR4JA (k_info, rate) code in makeH row
@@ -104,7 +148,7 @@ class Engine:
 
     def __init__(self, k_info: int = 1024, rate: str = "1/2", max_iter: int = 200, beta: float = 0.9375,
                  profile="T", device: int = 0, matrix=None, n_punct: int = 0, l_max: float = 0.0,
-                 r_max: float = 0.0, l_post_max: float = 0.0, gamma: float = 0.0):
+                 r_max: float = 0.0, l_post_max: float = 0.0, gamma: float = 0.0, variant: int = 0):
         self._lib = L.load()
         self._h = None
         prof = PROFILES[profile] if isinstance(profile, str) else int(profile)
@@ -118,7 +162,7 @@ class Engine:
         else:
             num, den = RATE_FRACTIONS[rate]
             desc = L.CodeDesc(L.LDPC_CODE_AR4JA, k_info, num, den)
-        cfg = L.DecoderCfg(prof, max_iter, beta, l_max, r_max, l_post_max, gamma)
+        cfg = L.DecoderCfg(prof, max_iter, beta, l_max, r_max, l_post_max, gamma, variant)
         h = C.c_void_p()
         rc = self._lib.ldpc_create(C.byref(desc), C.byref(cfg), device, C.byref(h))
         if rc != L.LDPC_OK:
@@ -128,7 +172,7 @@ class Engine:
         self._check(self._lib.ldpc_get_code_info(self._h, C.byref(info)))
         self.n, self.m, self.k_info, self.n_tx = info.n, info.m, info.k_info, info.n_tx
         self.z, self.n_layers, self.n_edges = info.z, info.n_layers, info.n_edges
-        self.max_iter, self.beta, self.rate, self.profile = max_iter, beta, rate, prof
+        self.max_iter, self.beta, self.rate, self.profile, self.variant = max_iter, beta, rate, prof, variant
 
     def _check(self, rc: int) -> None:
         if rc != L.LDPC_OK:
@@ -158,7 +202,8 @@ class Engine:
         return mu.value, sg.value
 
     def schedule(self):
-        return compiled_schedule(self.k_info, self.rate)
+        order = L.variant_fields(self.variant)[1] if self.variant else 0
+        return compiled_schedule(self.k_info, self.rate, order)
 
 
     @property
@@ -209,3 +254,69 @@ class Engine:
         out = np.empty((n_cw, self.n), dtype=np.uint16)
         self._check(self._lib.ldpc_gen_channel(self._h, seed, first_cw, n_cw, ebn0_db, out.ctypes.data))
         return out
+
+
+class Batch:
+    """Several AR4JA codes (Profile T) simulated together: every round is one
+    captured HIP graph holding each code's persistent kernel.
+
+    ``codes``: list of (k_info, rate); ``variants``: optional list of
+    LDPC_VARIANT codes (0 = library default).
+    """
+
+    def __init__(self, codes, max_iter: int = 200, beta: float = 0.9375, variants=None, device: int = 0):
+        self._lib = L.load()
+        self._h = None
+        self.codes = list(codes)
+        n = len(self.codes)
+        descs = (L.CodeDesc * n)(*[ar4ja_desc(k, r) for k, r in self.codes])
+        var = np.asarray(variants if variants is not None else [0] * n, dtype=np.uint32)
+        cfg = L.DecoderCfg(L.LDPC_PROFILE_T, max_iter, beta)
+        h = C.c_void_p()
+        rc = self._lib.ldpc_batch_create(descs, var.ctypes.data, n, C.byref(cfg), device, C.byref(h))
+        if rc != L.LDPC_OK:
+            raise L.LdpcError(rc, self._lib.ldpc_batch_last_error(None).decode())
+        self._h = h
+        self.max_iter = max_iter
+        self.info = []
+        for i in range(n):
+            ci = L.CodeInfo()
+            self._check(self._lib.ldpc_batch_code_info(self._h, i, C.byref(ci)))
+            self.info.append(ci)
+
+    def _check(self, rc: int) -> None:
+        if rc != L.LDPC_OK:
+            raise L.LdpcError(rc, self._lib.ldpc_batch_last_error(self._h).decode())
+
+    def run(self, ebn0_db, max_blocks, target_fe=0, seed=0x5EED2026, first_cw=0, chunk_cw=1 << 20):
+        """Per-code lists (or scalars for all codes) -> list of PointResult."""
+        n = len(self.codes)
+
+        def per(x):
+            return list(x) if isinstance(x, (list, tuple, np.ndarray)) else [x] * n
+
+        eb, mb, tf, sd, fc, ch = map(per, (ebn0_db, max_blocks, target_fe, seed, first_cw, chunk_cw))
+        pts = (L.PointCfg * n)(*[L.PointCfg(eb[i], sd[i], fc[i], mb[i], tf[i], ch[i]) for i in range(n)])
+        res = (L.PointResult * n)()
+        hist = np.zeros((n, self.max_iter + 1), dtype=np.uint64)
+        self._check(self._lib.ldpc_batch_run(self._h, pts, res, hist.ctypes.data))
+        return [PointResult(eb[i], r.blocks, r.frame_errors, r.bit_errors, r.iter_sum, r.next_cw, r.elapsed_ns * 1e-9,
+                            r.kernel_ns * 1e-9, r.launches, hist[i], self.info[i].n_tx, self.info[i].k_info)
+                for i, r in enumerate(res)]
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.ldpc_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

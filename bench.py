@@ -9,6 +9,11 @@ generation -> layered decoding -> FER/BER/iteration tallies, all on device
 (the launch shape of the error-floor configuration).  Synthetic data: the
 all-zero codeword plus AWGN from the counter-based stream.
 
+``--workload batch9`` measures the mixed-code extension instead: all nine
+AR4JA codes (K = 1024, 4096, 16384 x rate 1/2, 2/3, 4/5), equal codewords per
+code, each at its own Eb/N0 in [1, 3] dB, one captured HIP graph per step
+holding every code's persistent kernel.
+
 Prints ONE JSON line.  value = coded channel symbols per second over the timed
 wall clock (Gsps).  roofline = VALU issue (the decoder keeps each codeword on
 chip; HBM carries only counters), from HIP-event kernel time measured on the
@@ -45,6 +50,30 @@ LDS_BYTES_PER_CWI = 7680 * 10
 # LDS 128 B/clk/CU; HBM3E 8 TB/s.
 PEAK_VALU_TOPS = 256 * 128 * 2.4e9 / 1e12
 PEAK_LDS_TBS = 256 * 128 * 2.4e9 / 1e12
+
+
+# Mixed-code batch: (K, rate) -> Eb/N0 in each code's waterfall, in [1, 3] dB.
+BATCH9_EBN0 = {(1024, "1/2"): 2.0, (1024, "2/3"): 2.6, (1024, "4/5"): 3.0,
+               (4096, "1/2"): 1.5, (4096, "2/3"): 2.2, (4096, "4/5"): 3.0,
+               (16384, "1/2"): 1.2, (16384, "2/3"): 2.0, (16384, "4/5"): 3.0}
+BATCH9_CW_PER_CODE = 1 << 15
+DEGREE = {"1/2": 6, "2/3": 10, "4/5": 18}
+
+
+def ops_per_check(d: int) -> int:
+    """VALU lane-ops of one Profile T check update of degree d (counting rules:
+    q 5/edge, sign parity d-1, min* 11 each, beta / copysign / parity 1 each
+    per edge, RNE 4/edge, L' 5/edge).  min* evaluations: 3 for d = 3, 6h - 6
+    for d = 2h (pairs, prefix, suffix, pair-others, outputs).  d = 6 gives 239,
+    d = 3 gives 86."""
+    n_amin = 3 if d == 3 else 6 * (d // 2) - 6
+    return 18 * d - 1 + 11 * n_amin
+
+
+def ops_per_cwi(k: int, rate: str) -> int:
+    """Lane-ops of one codeword-iteration: M checks each of degree 3, d, d."""
+    m = {(1024, "1/2"): 512, (1024, "2/3"): 256, (1024, "4/5"): 128}[(1024, rate)] * (k // 1024)
+    return m * (ops_per_check(3) + 2 * ops_per_check(DEGREE[rate]))
 
 
 def poisson_ci95(k: int):
@@ -130,6 +159,88 @@ def copy_calibration(mib: int, reps: int = 3) -> None:
     print(json.dumps({"copy_calibration_bytes_per_copy": n, "copies": reps}), flush=True)
 
 
+def cpu_oracle_batch9(seconds: float):
+    """CPU oracle on a sample of every code of the batch (same Eb/N0 table)."""
+    import oracle as O
+    from ldpc_mi355x import codes
+    threads = O.num_threads()
+    syms, t0, rounds = 0, time.time(), 0
+    while time.time() - t0 < seconds:
+        for (k, rate), eb in BATCH9_EBN0.items():
+            code = codes.ar4ja(k, rate)
+            g = 10 ** (eb / 10)
+            r = code.k / code.n_tx
+            n = max(threads, 4 * threads * 1024 // k)
+            O.run_point_t(code, 0xC0DE, rounds * n, n, 4 * r * g, math.sqrt(8 * r * g), MAX_ITER, BETA, threads)
+            syms += n * code.n_tx
+        rounds += 1
+    dt = time.time() - t0
+    return {"value": syms / dt / 1e9, "unit": "Gsps", "cores": threads, "kind": "port",
+            "sample": f"CPU oracle (oracle/ldpc_oracle.c, Profile T) on all nine codes at the batch's Eb/N0, "
+                      f"{rounds} round(s), {syms} symbols, 200 max iter, {threads} OpenMP threads"}
+
+
+def bench_batch9(args) -> None:
+    from ldpc_mi355x import decoder
+    codes9 = list(BATCH9_EBN0)
+    ebn0 = [BATCH9_EBN0[c] for c in codes9]
+    n = BATCH9_CW_PER_CODE
+    b = decoder.Batch(codes9, max_iter=MAX_ITER, beta=BETA)
+    seed, first = 0x5EED2026, 0
+    for _ in range(args.warmup):
+        b.run(ebn0, n, seed=seed, first_cw=first, chunk_cw=n)
+        first += n
+    acc = None
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        res = b.run(ebn0, n, seed=seed, first_cw=first, chunk_cw=n)
+        first += n
+        if acc is None:
+            acc = res
+        else:
+            for a, r in zip(acc, res):
+                a.blocks += r.blocks
+                a.frame_errors += r.frame_errors
+                a.bit_errors += r.bit_errors
+                a.iter_sum += r.iter_sum
+                a.kernel_s += r.kernel_s
+                a.launches += r.launches
+                a.hist = a.hist + r.hist
+    wall = time.perf_counter() - t0
+    syms = sum(a.blocks * a.n_tx for a in acc)
+    graph_s = acc[0].kernel_s  # every code takes part in every round
+    ops = sum(ops_per_cwi(k, r) * a.iter_sum for (k, r), a in zip(codes9, acc))
+    ops_tops = ops / graph_s / 1e12
+    cpu = None
+    if not args.no_cpu:
+        try:
+            cpu = cpu_oracle_batch9(args.cpu_seconds)
+        except Exception as e:
+            cpu = {"value": None, "unit": "Gsps", "cores": os.cpu_count(), "kind": "port", "sample": f"failed: {e}"}
+    per_code = {f"K{k} r{r}": {"ebn0_db": eb, "fer": a.fer, "frame_errors": a.frame_errors, "ber": a.ber,
+                                "mean_iterations": a.mean_iterations, "n_tx": a.n_tx}
+                for (k, r), eb, a in zip(codes9, ebn0, acc)}
+    out = {
+        "metric": "Gsps (channel sym/s) 1 GPU, nine AR4JA codes in one mixed batch",
+        "value": syms / wall / 1e9, "unit": "Gsps", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "bf16",
+        "data": "synthetic: all-zero codeword, BPSK, AWGN from Philox4x32-10 + Box-Muller generated on device",
+        "config": {"workload": f"nine AR4JA codes (K 1024/4096/16384 x r1/2, 2/3, 4/5), {n} codewords of each "
+                               f"per step, per-code Eb/N0 in [1, 3] dB, 200 max iterations, Profile T, one "
+                               f"captured HIP graph per step", "codewords_per_code_per_step": n,
+                   "max_iter": MAX_ITER, "beta": BETA},
+        "roofline": {"bound": "valu", "achieved": ops_tops, "peak": PEAK_VALU_TOPS, "unit": "Tlane-op/s",
+                     "frac": ops_tops / PEAK_VALU_TOPS, "traffic": None, "kernel": "k_multi_* (graph of nine)",
+                     "graph_ms_per_step": graph_s / args.steps * 1e3},
+        "cpu_baseline": cpu,
+        "per_code": per_code,
+        "speedup_vs_cpu": (syms / wall / 1e9 / cpu["value"]) if cpu and cpu.get("value") else None,
+    }
+    print(json.dumps(out), flush=True)
+    b.close()
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -139,6 +250,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run device-to-device copies of MIB MiB (counter calibration)")
+    ap.add_argument("--workload", choices=["r12k1024", "batch9"], default="r12k1024",
+                    help="r12k1024: the headline AR4JA r1/2 K=1024 point; batch9: the nine-code mixed batch")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
@@ -148,6 +261,9 @@ def main():
               "but no multi-GPU figure is produced); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
+    if args.workload == "batch9":
+        bench_batch9(args)
+        return
     from ldpc_mi355x import decoder
     eng = decoder.Engine(1024, "1/2", max_iter=MAX_ITER, beta=BETA, device=0)
     seed = 0x5EED2026

@@ -232,10 +232,13 @@ const MultiVariant *find_variant(int k_id, int rate_id, uint32_t variant) {
     return nullptr;
 }
 
-// Default variant per code (layer order 0): checks per thread from
-// ldpc_autotune runs on an MI355X at each code's waterfall Eb/N0.
+// Default variant per code (layer order 0): the checks per thread that
+// ldpc_autotune found fastest on an MI355X in each code's waterfall
+// (K = 1024 / 4096 / 16384 rows; rates 1/2, 2/3, 4/5 columns).  One check per
+// thread wins wherever it exists except K = 4096 r1/2; K = 16384 r1/2
+// (z = 2048) has two-check threads only.
 uint32_t default_variant(int k_id, int rate_id) {
-    static const int C[3][3] = {{2, 1, 1}, {2, 1, 1}, {2, 1, 1}};
+    static const int C[3][3] = {{1, 1, 1}, {2, 1, 1}, {2, 1, 1}};
     return LDPC_VARIANT(C[k_id][rate_id], 0);
 }
 

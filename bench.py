@@ -9,7 +9,11 @@ generation -> layered decoding -> FER/BER/iteration tallies, all on device
 (the launch shape of the error-floor configuration).  Synthetic data: the
 all-zero codeword plus AWGN from the counter-based stream.
 
-``--workload batch9`` measures the mixed-code extension instead: all nine
+``--workload scalar`` measures Profile R (the scalar_5w arithmetic that
+run_simulation.py drives) on the reference's own .chinn.out matrix (AR4JA
+r1/2 K=1024 relabelled, rows in file order), 2.0 dB, 200 max iterations --
+the configuration of the reference's published 158 Msps scalar figure
+(README.md:108).  ``--workload batch9`` measures the mixed-code extension: all nine
 AR4JA codes (K = 1024, 4096, 16384 x rate 1/2, 2/3, 4/5), equal codewords per
 code, each at its own Eb/N0 in [1, 3] dB, one captured HIP graph per step
 holding every code's persistent kernel.
@@ -180,6 +184,75 @@ def cpu_oracle_batch9(seconds: float):
                       f"{rounds} round(s), {syms} symbols, 200 max iter, {threads} OpenMP threads"}
 
 
+SCALAR_MATRIX = os.path.join(ROOT, "tests", "golden", "AR4JA_r45_4c_128c_r12.chinn.out")
+
+
+def cpu_oracle_scalar(rows, n, seconds: float):
+    import oracle as O
+    threads = O.num_threads()
+    g = 10 ** (EBN0_DB / 10)
+    mu, sg = 4 * 0.5 * g, math.sqrt(8 * 0.5 * g)
+    blocks, t0 = 0, time.time()
+    while time.time() - t0 < seconds:
+        nb = 256 * threads
+        O.run_point_r(n, rows, N_TX, 0xC0DE, blocks, nb, mu, sg, max_iter=MAX_ITER, threads=threads)
+        blocks += nb
+    dt = time.time() - t0
+    return {"value": blocks * N_TX / dt / 1e9, "unit": "Gsps", "cores": threads, "kind": "port",
+            "sample": f"CPU oracle (oracle/ldpc_oracle.c, Profile R) on the .chinn.out, {blocks} codewords at "
+                      f"{EBN0_DB} dB, 200 max iter, {threads} OpenMP threads"}
+
+
+def bench_scalar(args) -> None:
+    from ldpc_mi355x import codes, decoder
+    m, n, rows = codes.read_chinn_out(SCALAR_MATRIX)
+    eng = decoder.Engine(profile="R", matrix=(rows, n), n_punct=n - N_TX, max_iter=MAX_ITER)
+    seed, first = 133742, 0
+    for _ in range(args.warmup):
+        first = eng.run_point(EBN0_DB, CW_PER_STEP, seed=seed, first_cw=first, chunk_cw=CW_PER_STEP).next_cw
+    eng.sync()
+    acc = None
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        acc = eng.run_point(EBN0_DB, CW_PER_STEP, seed=seed, first_cw=first, chunk_cw=CW_PER_STEP, result=acc)
+        first = acc.next_cw
+    eng.sync()
+    wall = time.perf_counter() - t0
+    value = acc.blocks * N_TX / wall / 1e9
+    per_launch_s = acc.kernel_s / acc.launches
+    # Profile R per edge: q 1, |q| and sign 2, min/argmin 2, fold step 9, output select/sign 3, two RNE 8,
+    # L' add 1 -> about 26 lane-ops; 7680 edges per codeword-iteration
+    ops = 26 * 7680 * acc.iter_sum / acc.launches
+    cpu = None
+    if not args.no_cpu:
+        try:
+            cpu = cpu_oracle_scalar(rows, n, args.cpu_seconds)
+        except Exception as e:
+            cpu = {"value": None, "unit": "Gsps", "cores": os.cpu_count(), "kind": "port", "sample": f"failed: {e}"}
+    fe_lo, fe_hi = poisson_ci95(acc.frame_errors)
+    out = {
+        "metric": "Gsps (channel sym/s) 1 GPU, AR4JA r1/2 K=1024 @2.0dB, Profile R (scalar_5w arithmetic)",
+        "value": value, "unit": "Gsps", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "bf16",
+        "data": "synthetic: all-zero codeword, BPSK, AWGN from Philox4x32-10 + Box-Muller generated on device",
+        "config": {"workload": "Profile R on matrices/AR4JA_r45_4c_128c_r12.chinn.out (rows in file order, 37 "
+                               "levels), 512 punctured, Eb/N0 2.0 dB, 200 max iterations, 2^20 codewords per step",
+                   "max_iter": MAX_ITER, "codewords_per_step": CW_PER_STEP},
+        "roofline": {"bound": "valu", "achieved": ops / per_launch_s / 1e12, "peak": PEAK_VALU_TOPS,
+                     "unit": "Tlane-op/s", "frac": ops / per_launch_s / 1e12 / PEAK_VALU_TOPS, "traffic": None,
+                     "kernel": "k_monte_carlo_r", "kernel_ms_per_launch": per_launch_s * 1e3},
+        "cpu_baseline": cpu,
+        "fer": acc.fer, "fer_ci95": [fe_lo / acc.blocks, fe_hi / acc.blocks], "frame_errors": acc.frame_errors,
+        "ber": acc.bit_errors / (acc.blocks * N_TX), "mean_iterations": acc.mean_iterations, "blocks": acc.blocks,
+        "kernel_gsps": acc.blocks * N_TX / acc.kernel_s / 1e9,
+        "reference_published_gsps": 0.158,
+        "speedup_vs_cpu": (value / cpu["value"]) if cpu and cpu.get("value") else None,
+    }
+    print(json.dumps(out), flush=True)
+    eng.close()
+
+
 def bench_batch9(args) -> None:
     from ldpc_mi355x import decoder
     codes9 = list(BATCH9_EBN0)
@@ -250,8 +323,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run device-to-device copies of MIB MiB (counter calibration)")
-    ap.add_argument("--workload", choices=["r12k1024", "batch9"], default="r12k1024",
-                    help="r12k1024: the headline AR4JA r1/2 K=1024 point; batch9: the nine-code mixed batch")
+    ap.add_argument("--workload", choices=["r12k1024", "scalar", "batch9"], default="r12k1024",
+                    help="r12k1024: the headline AR4JA r1/2 K=1024 point; scalar: Profile R on the .chinn.out; "
+                         "batch9: the nine-code mixed batch")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
@@ -263,6 +337,9 @@ def main():
 
     if args.workload == "batch9":
         bench_batch9(args)
+        return
+    if args.workload == "scalar":
+        bench_scalar(args)
         return
     from ldpc_mi355x import decoder
     eng = decoder.Engine(1024, "1/2", max_iter=MAX_ITER, beta=BETA, device=0)

@@ -1,0 +1,144 @@
+"""Differentiable min-sum decoder on the GPU (csrc/dmin_torch.hip): forward
+bit for bit against the oracle, backward against the PyTorch-autograd
+restatement and against float64 central finite differences, checkpointing
+that changes nothing, and a short training run."""
+import numpy as np
+import pytest
+import torch
+
+import dmin_oracle as DO
+import dmin_reference as REF
+from conftest import channel_params
+from ldpc_mi355x import codes
+
+pytestmark = pytest.mark.gpu
+
+
+def dmin():
+    from ldpc_mi355x import dmin as D
+    return D
+
+
+def llr_batch(code, B, ebn0, seed):
+    rng = np.random.default_rng(seed)
+    mu, sg = channel_params(ebn0, code.k / code.n_tx)
+    x = np.zeros((B, code.n), np.float32)
+    x[:, :code.n_tx] = (mu + sg * rng.standard_normal((B, code.n_tx))).astype(np.float32)
+    return x
+
+
+def rand_params(code, G, seed):
+    rng = np.random.default_rng(seed)
+    P = DO.n_params(code)
+    return (rng.uniform(0.7, 1.0, (G, P)).astype(np.float32), rng.uniform(0.05, 0.3, (G, P)).astype(np.float32))
+
+
+def run_ext(code, llr, a, b, T, corr, ckpt_every=4, grad=True, r_max=0.0):
+    D = dmin()
+    tab = D.schedule_table(code)
+    x = torch.tensor(llr, device="cuda", requires_grad=grad)
+    at = torch.tensor(a, device="cuda", requires_grad=grad)
+    bt = torch.tensor(b, device="cuda", requires_grad=grad)
+    L = D.DMinSumFunction.apply(x, at, bt, tab, T, ckpt_every, corr, r_max)
+    if grad:
+        D.soft_bce(L, code.n_tx).backward()
+        torch.cuda.synchronize()
+        return L.detach().cpu().numpy(), x.grad.cpu().numpy(), at.grad.cpu().numpy(), bt.grad.cpu().numpy()
+    return L.detach().cpu().numpy()
+
+
+@pytest.mark.parametrize("k,rate,B,T", [(1024, "1/2", 64, 4), (1024, "4/5", 32, 3), (4096, "2/3", 8, 2),
+                                        (16384, "1/2", 4, 2)])
+@pytest.mark.parametrize("corr,r_max", [(False, 0.0), (True, 0.0), (False, 3.0)])
+def test_forward_matches_oracle_bit_for_bit(k, rate, B, T, corr, r_max):
+    code = codes.ar4ja(k, rate)
+    llr = llr_batch(code, B, 1.0 + 0.8 * (rate != "1/2"), 3)
+    a, b = rand_params(code, T, 4)
+    print(f"forward K={k} rate={rate} B={B} T={T} corr={corr} r_max={r_max}", flush=True)
+    got = run_ext(code, llr, a, b, T, corr, grad=False, r_max=r_max)
+    exp = DO.forward(code, llr, a, b, T, corr, r_max=r_max)
+    np.testing.assert_array_equal(got.view(np.uint32), exp.view(np.uint32))
+
+
+@pytest.mark.parametrize("corr,r_max", [(False, 0.0), (True, 0.0), (True, 2.5)])
+@pytest.mark.parametrize("G", [1, 5])
+def test_backward_matches_torch_autograd(corr, r_max, G):
+    code = codes.ar4ja(1024, "1/2")
+    T = 5
+    llr = llr_batch(code, 48, 1.4, 7)
+    a, b = rand_params(code, G, 8)
+    L, gx, ga, gb = run_ext(code, llr, a, b, T, corr, r_max=r_max)
+    x = torch.tensor(llr, device="cuda", requires_grad=True)
+    at = torch.tensor(a, device="cuda", requires_grad=True)
+    bt = torch.tensor(b, device="cuda", requires_grad=True)
+    Lr = REF.forward(code, x, at, bt, T, corr, r_max)
+    REF.loss(Lr, code.n_tx).backward()
+    np.testing.assert_array_equal(L.view(np.uint32), Lr.detach().cpu().numpy().view(np.uint32))
+    np.testing.assert_allclose(gx, x.grad.cpu().numpy(), rtol=1e-4, atol=1e-9)
+    np.testing.assert_allclose(ga, at.grad.cpu().numpy(), rtol=1e-4, atol=1e-8)
+    np.testing.assert_allclose(gb, bt.grad.cpu().numpy(), rtol=1e-4, atol=1e-8)
+    assert np.abs(ga).max() > 0 and np.abs(gb).max() > 0
+
+
+def test_gradient_against_float64_finite_differences():
+    """64 codewords, T = 20, tied parameters, central differences of the
+    float64 oracle with step 1e-3.
+
+    The operating point is 2.0 dB with messages capped at 8, where every
+    codeword of the batch converges.  A codeword that fails makes the
+    unrolled decoder chaotic: argmin and sign decisions flip under 1e-3
+    perturbations and float32 / float64 trajectories part, so a difference
+    quotient there measures nothing (the pathwise gradient is still what the
+    PyTorch-autograd test above pins)."""
+    code = codes.ar4ja(1024, "1/2")
+    T = 20
+    llr = llr_batch(code, 64, 2.0, 11)
+    a, b = rand_params(code, 1, 12)
+    L, _, ga, gb = run_ext(code, llr, a, b, T, False, r_max=8.0)
+    assert ((L[:, :code.n_tx] <= 0).sum(1) == 0).all(), "every codeword of the batch converges"
+    idx = [(0, p) for p in (0, 5, 11, 17, 26, 33, 41, 52, 59)]
+    an, fd = [], []
+    for which, g in (("alpha", ga), ("beta", gb)):
+        f = DO.fd_grad(code, llr, a, b, T, which, idx, 1e-3, r_max=8.0)
+        for k, (gg, p) in enumerate(idx):
+            an.append(float(g[gg, p]))
+            fd.append(float(f[k]))
+            print(which, p, an[-1], fd[-1], flush=True)
+    an, fd = np.array(an), np.array(fd)
+    # entries far below the largest sit at float32's summation noise floor,
+    # so the error is measured against the gradient's scale
+    err = np.abs(an - fd).max() / np.abs(fd).max()
+    cos = an @ fd / (np.linalg.norm(an) * np.linalg.norm(fd))
+    print("max error / max |fd|", err, "cos", cos, flush=True)
+    assert np.abs(fd).max() > 0
+    assert cos > 0.9999 and err < 1e-3
+
+
+def test_checkpoint_interval_changes_nothing():
+    code = codes.ar4ja(1024, "1/2")
+    T = 6
+    llr = llr_batch(code, 32, 1.3, 21)
+    a, b = rand_params(code, 1, 22)
+    runs = [run_ext(code, llr, a, b, T, True, ckpt_every=k) for k in (1, 4, 12, T * 12)]
+    for r in runs[1:]:
+        for x, y in zip(runs[0], r):
+            np.testing.assert_array_equal(x, y)
+
+
+def test_training_steps_run():
+    D = dmin()
+    torch.manual_seed(0)
+    model = D.DMinSum(1024, "1/2", T=10, ckpt_every=12)
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(5)
+    eb = torch.full((4096,), 1.5, device="cuda")
+    llr = D.channel_llr(model.code, eb, gen)
+    with torch.no_grad():
+        before = float(D.soft_bce(model(llr), model.n_tx))
+    log = D.train(model, steps=30, batch=4096, micro_batch=2048, lr=1e-2, ebn0_range=(1.5, 1.5), seed=1,
+                  log_every=10)
+    with torch.no_grad():
+        after = float(D.soft_bce(model(llr), model.n_tx))
+    print("loss", before, "->", after, [(l.step, l.loss) for l in log], flush=True)
+    assert np.isfinite(after) and all(np.isfinite(l.loss) for l in log)
+    assert float((model.alpha - 1).abs().max()) > 0 and float(model.beta.abs().max()) > 0

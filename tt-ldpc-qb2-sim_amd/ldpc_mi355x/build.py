@@ -28,6 +28,11 @@ MULTI_CODES = [(k, r) for k in range(3) for r in range(3)]
 CLIS = {"ldpc_sim_scalar": "cli_scalar.cpp", "ldpc_sim_circulant": "cli_circulant.cpp"}
 HEADERS = ["ar4ja_schedule.h", "device_math.h", "kernels.h", "runtime_internal.h"]
 OBJ_DIR = os.path.join(PKG, "_build")
+# PyTorch extension (differentiable decoder): its own module, loaded into the
+# torch process so that its kernels run on torch's HIP runtime and streams
+TORCH_EXT = "ldpc_dmin"
+TORCH_EXT_SRC = "dmin_torch.hip"
+TORCH_EXT_LIB = os.path.join(PKG, TORCH_EXT + ".so")
 
 HIPCC_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-gpu-rdc",
@@ -82,6 +87,31 @@ def build(force: bool = True, verbose: bool = False, jobs: int = 0) -> str:
     return LIB
 
 
+def build_torch_ext(verbose: bool = False) -> str:
+    """Compile csrc/dmin_torch.hip against the installed PyTorch (plain hipcc,
+    torch's include and library paths) into ldpc_mi355x/ldpc_dmin.so."""
+    import sysconfig
+
+    import torch
+    from torch.utils import cpp_extension as ce
+    inc = ce.include_paths(device_type="cuda") + [sysconfig.get_paths()["include"]]
+    torch_lib = os.path.join(os.path.dirname(torch.__file__), "lib")
+    abi = int(torch._C._GLIBCXX_USE_CXX11_ABI)
+    tmp = TORCH_EXT_LIB + ".tmp"
+    cmd = [hipcc(), f"--offload-arch={ARCH}", *HIPCC_FLAGS, "-shared", "-Wno-unused-result",
+           "-D__HIP_PLATFORM_AMD__=1", "-DUSE_ROCM=1", "-DHIPBLAS_V2", "-D__HIP_NO_HALF_OPERATORS__=1",
+           "-D__HIP_NO_HALF_CONVERSIONS__=1", f"-D_GLIBCXX_USE_CXX11_ABI={abi}",
+           f"-DTORCH_EXTENSION_NAME={TORCH_EXT}", "-DTORCH_API_INCLUDE_EXTENSION_H",
+           *[f"-I{d}" for d in inc], os.path.join(CSRC, TORCH_EXT_SRC),
+           f"-L{torch_lib}", "-lc10", "-lc10_hip", "-ltorch", "-ltorch_cpu", "-ltorch_hip", "-ltorch_python",
+           f"-Wl,-rpath,{torch_lib}", "-o", tmp]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True, cwd=CSRC)
+    os.replace(tmp, TORCH_EXT_LIB)
+    return TORCH_EXT_LIB
+
+
 def build_clis(verbose: bool = False) -> None:
     inc = os.path.join(os.path.dirname(ROOT), "include")
     for exe, src in CLIS.items():
@@ -95,3 +125,4 @@ def build_clis(verbose: bool = False) -> None:
 
 if __name__ == "__main__":
     print(build(force=True, verbose="-v" in sys.argv))
+    print(build_torch_ext(verbose="-v" in sys.argv))

@@ -24,11 +24,12 @@ def layers_of(code, device):
     return out, e_off
 
 
-def forward(code, llr, alpha, beta, T, corr=False, r_max=0.0):
+def forward(code, llr, alpha, beta, T, corr=False, r_max=0.0, all_iters=False, trunc=0):
     lays, E = layers_of(code, llr.device)
     B = llr.shape[0]
     L = llr
     R = torch.zeros(B, E, dtype=llr.dtype, device=llr.device)
+    outs = []
     for t in range(T):
         g = t if alpha.shape[0] == T else 0
         for deg, z, var, e_off, p_off in lays:
@@ -62,8 +63,11 @@ def forward(code, llr, alpha, beta, T, corr=False, r_max=0.0):
             Lnew = q + r
             L = L.scatter(1, var.reshape(1, -1).expand(B, -1), Lnew.reshape(B, -1))
             R = torch.cat([R[:, :e_off], r.reshape(B, -1), R[:, e_off + deg * z:]], dim=1)
-    return L
+        outs.append(L)
+        if trunc and (t + 1) % trunc == 0 and t < T - 1:
+            L, R = L.detach(), R.detach()
+    return torch.stack(outs) if all_iters else L
 
 
 def loss(L, n_tx):
-    return torch.nn.functional.softplus(-L[:, :n_tx]).mean()
+    return torch.nn.functional.softplus(-L[..., :n_tx]).mean()

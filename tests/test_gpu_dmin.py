@@ -114,6 +114,33 @@ def test_gradient_against_float64_finite_differences():
     assert cos > 0.9999 and err < 1e-3
 
 
+@pytest.mark.parametrize("trunc", [0, 1, 3])
+def test_per_iteration_outputs_gradient_matches_torch_autograd(trunc):
+    """Multi-loss: the BCE averaged over every iteration's soft output, with
+    backpropagation through all iterations or truncated every `trunc`."""
+    D = dmin()
+    code = codes.ar4ja(1024, "1/2")
+    T = 4
+    llr = llr_batch(code, 32, 1.4, 31)
+    a, b = rand_params(code, 1, 32)
+    tab = D.schedule_table(code)
+    x = torch.tensor(llr, device="cuda", requires_grad=True)
+    at = torch.tensor(a, device="cuda", requires_grad=True)
+    bt = torch.tensor(b, device="cuda", requires_grad=True)
+    Lall = D.DMinSumFunction.apply(x, at, bt, tab, T, 5, True, 0.0, True, trunc)
+    assert tuple(Lall.shape) == (T, 32, code.n)
+    D.soft_bce(Lall, code.n_tx).backward()
+    x2 = torch.tensor(llr, device="cuda", requires_grad=True)
+    a2 = torch.tensor(a, device="cuda", requires_grad=True)
+    b2 = torch.tensor(b, device="cuda", requires_grad=True)
+    Lr = REF.forward(code, x2, a2, b2, T, True, 0.0, all_iters=True, trunc=trunc)
+    REF.loss(Lr, code.n_tx).backward()
+    np.testing.assert_array_equal(Lall.detach().cpu().numpy(), Lr.detach().cpu().numpy())
+    np.testing.assert_allclose(x.grad.cpu().numpy(), x2.grad.cpu().numpy(), rtol=1e-4, atol=1e-9)
+    np.testing.assert_allclose(at.grad.cpu().numpy(), a2.grad.cpu().numpy(), rtol=1e-4, atol=1e-8)
+    np.testing.assert_allclose(bt.grad.cpu().numpy(), b2.grad.cpu().numpy(), rtol=1e-4, atol=1e-8)
+
+
 def test_checkpoint_interval_changes_nothing():
     code = codes.ar4ja(1024, "1/2")
     T = 6

@@ -298,9 +298,11 @@ void check_inputs(const at::Tensor &llr, const at::Tensor &alpha, const at::Tens
     TORCH_CHECK(T >= 1, "T >= 1");
 }
 
-// Returns [L_out, ckL_0, ckR_0, ckL_1, ckR_1, ...] (checkpoints only if keep).
+// Returns [out, ckL_0, ckR_0, ckL_1, ckR_1, ...] (checkpoints only if keep);
+// out is L after the last iteration [B, N], or after every iteration
+// [T, B, N] if all_iters.
 std::vector<at::Tensor> dmin_forward(at::Tensor llr, at::Tensor alpha, at::Tensor beta, at::Tensor tab, int64_t T,
-                                     int64_t ckpt_every, bool corr, bool keep, double r_max) {
+                                     int64_t ckpt_every, bool corr, bool keep, double r_max, bool all_iters) {
     const Sched s = make_sched(tab, llr.size(1));
     check_inputs(llr, alpha, beta, s, T);
     TORCH_CHECK(ckpt_every >= 1, "ckpt_every >= 1");
@@ -308,7 +310,8 @@ std::vector<at::Tensor> dmin_forward(at::Tensor llr, at::Tensor alpha, at::Tenso
     hipStream_t st = c10::hip::getCurrentHIPStream(llr.device().index()).stream();
     at::Tensor L = llr.clone();
     at::Tensor R = at::zeros({B, s.e}, llr.options());
-    std::vector<at::Tensor> out{L};
+    at::Tensor all = all_iters ? at::empty({T, B, (int64_t)s.n}, llr.options()) : at::Tensor();
+    std::vector<at::Tensor> out{all_iters ? all : L};
     const int nl = (int)s.layers.size();
     for (int64_t step = 0; step < T * nl; ++step) {
         if (keep && step % ckpt_every == 0) {
@@ -319,23 +322,32 @@ std::vector<at::Tensor> dmin_forward(at::Tensor llr, at::Tensor alpha, at::Tenso
         const LayerArgs &la = s.layers[step % nl];
         LDPC_DISPATCH(la.deg, corr, fwd_launch, L.data_ptr<float>(), R.data_ptr<float>(), nullptr,
                       alpha.data_ptr<float>() + g * s.p, beta.data_ptr<float>() + g * s.p, la, (float)r_max, B, st);
+        if (all_iters && step % nl == nl - 1) all[step / nl].copy_(L);
     }
     TORCH_CHECK(hipGetLastError() == hipSuccess, "dmin forward launch failed");
     return out;
 }
 
+// grad_out: [B, N] (adjoint of the last iteration's L) or [T, B, N] (of every
+// iteration's L).  trunc > 0 truncates backpropagation through iterations:
+// the adjoints of L and R are cut at every iteration boundary that is a
+// multiple of trunc (the state there is treated as a constant), so each
+// window of trunc iterations only sees the losses of its own iterations.
 // Returns [grad_llr, grad_alpha, grad_beta].
 std::vector<at::Tensor> dmin_backward(at::Tensor grad_out, std::vector<at::Tensor> ckpts, at::Tensor alpha,
                                       at::Tensor beta, at::Tensor tab, int64_t T, int64_t ckpt_every, bool corr,
-                                      double r_max) {
-    const Sched s = make_sched(tab, grad_out.size(1));
-    check_inputs(grad_out.contiguous(), alpha, beta, s, T);
+                                      double r_max, int64_t trunc) {
+    const bool all_iters = grad_out.dim() == 3;
+    TORCH_CHECK(!all_iters || grad_out.size(0) == T, "per-iteration gradient must be [T, B, N]");
+    const at::Tensor g_last = all_iters ? grad_out[T - 1] : grad_out;
+    const Sched s = make_sched(tab, g_last.size(1));
+    check_inputs(g_last.contiguous(), alpha, beta, s, T);
     const int nl = (int)s.layers.size();
     const int64_t n_steps = T * nl, n_seg = (n_steps + ckpt_every - 1) / ckpt_every;
     TORCH_CHECK((int64_t)ckpts.size() == 2 * n_seg, "checkpoint list does not match T and ckpt_every");
-    const int64_t B = grad_out.size(0);
+    const int64_t B = g_last.size(0);
     hipStream_t st = c10::hip::getCurrentHIPStream(grad_out.device().index()).stream();
-    at::Tensor gL = grad_out.contiguous().clone();
+    at::Tensor gL = g_last.contiguous().clone();
     at::Tensor gR = at::zeros({B, s.e}, grad_out.options());
     at::Tensor ga = at::zeros_like(alpha), gb = at::zeros_like(beta);
     for (int64_t seg = n_seg - 1; seg >= 0; --seg) {
@@ -358,6 +370,15 @@ std::vector<at::Tensor> dmin_backward(at::Tensor grad_out, std::vector<at::Tenso
         for (int64_t step = s1 - 1; step >= s0; --step) {
             const int64_t g = alpha.size(0) == T ? step / nl : 0;
             const LayerArgs &la = s.layers[step % nl];
+            // entering iteration t from its end: cut at a truncation boundary,
+            // then add the adjoint of the iteration's output
+            if (step % nl == nl - 1 && step / nl < T - 1) {
+                if (trunc > 0 && (step / nl + 1) % trunc == 0) {
+                    gL.zero_();
+                    gR.zero_();
+                }
+                if (all_iters) gL.add_(grad_out[step / nl]);
+            }
             const int64_t nblk = (B * la.z + TPB - 1) / TPB;
             at::Tensor partial = at::empty({nblk, 2, (int64_t)la.deg}, grad_out.options());
             LDPC_DISPATCH(la.deg, corr, bwd_launch, gL.data_ptr<float>(), gR.data_ptr<float>(),

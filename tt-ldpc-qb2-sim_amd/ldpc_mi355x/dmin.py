@@ -55,37 +55,39 @@ def n_params(code) -> int:
 
 class DMinSumFunction(torch.autograd.Function):
     """(llr [B, N], alpha [G, P], beta [G, P]) -> soft output L [B, N] after T
-    iterations.  Gradients flow to all three inputs."""
+    iterations, or after every iteration [T, B, N] if all_iters.  Gradients
+    flow to all three inputs."""
 
     @staticmethod
-    def forward(ctx, llr, alpha, beta, tab, T, ckpt_every, corr, r_max=0.0):
+    def forward(ctx, llr, alpha, beta, tab, T, ckpt_every, corr, r_max=0.0, all_iters=False, trunc=0):
         need = ctx.needs_input_grad[0] or ctx.needs_input_grad[1] or ctx.needs_input_grad[2]
         outs = _ext.forward(llr.contiguous(), alpha.contiguous(), beta.contiguous(), tab, T, ckpt_every, corr, need,
-                            float(r_max))
+                            float(r_max), bool(all_iters))
         ctx.ckpts = outs[1:]
-        ctx.cfg = (tab, T, ckpt_every, corr, float(r_max))
+        ctx.cfg = (tab, T, ckpt_every, corr, float(r_max), int(trunc))
         ctx.save_for_backward(alpha, beta)
         return outs[0]
 
     @staticmethod
     def backward(ctx, grad_out):
         alpha, beta = ctx.saved_tensors
-        tab, T, ckpt_every, corr, r_max = ctx.cfg
+        tab, T, ckpt_every, corr, r_max, trunc = ctx.cfg
         g_llr, g_alpha, g_beta = _ext.backward(grad_out.contiguous(), list(ctx.ckpts), alpha.contiguous(),
-                                               beta.contiguous(), tab, T, ckpt_every, corr, r_max)
+                                               beta.contiguous(), tab, T, ckpt_every, corr, r_max, trunc)
         ctx.ckpts = None
-        return g_llr, g_alpha, g_beta, None, None, None, None, None
+        return g_llr, g_alpha, g_beta, None, None, None, None, None, None, None
 
 
 class DMinSum(torch.nn.Module):
     """Learnable layered min-sum decoder for one AR4JA code."""
 
     def __init__(self, k_info: int = 1024, rate: str = "1/2", T: int = 20, ckpt_every: int = 4,
-                 correction: bool = False, per_iteration: bool = False, r_max: float = 0.0, device="cuda"):
+                 correction: bool = False, per_iteration: bool = False, r_max: float = 0.0, trunc: int = 0,
+                 device="cuda"):
         super().__init__()
         self.code = codes.ar4ja(k_info, rate)
         self.k_info, self.rate, self.T = k_info, rate, T
-        self.ckpt_every, self.correction, self.r_max = ckpt_every, correction, r_max
+        self.ckpt_every, self.correction, self.r_max, self.trunc = ckpt_every, correction, r_max, trunc
         self.tab = schedule_table(self.code)
         p = n_params(self.code)
         g = T if per_iteration else 1
@@ -100,9 +102,10 @@ class DMinSum(torch.nn.Module):
     def n_tx(self) -> int:
         return self.code.n_tx
 
-    def forward(self, llr: torch.Tensor) -> torch.Tensor:
+    def forward(self, llr: torch.Tensor, all_iters: bool = False) -> torch.Tensor:
+        """Soft output after T iterations [B, N], or after each [T, B, N]."""
         return DMinSumFunction.apply(llr, self.alpha, self.beta, self.tab, self.T, self.ckpt_every,
-                                     self.correction, self.r_max)
+                                     self.correction, self.r_max, all_iters, self.trunc)
 
     def export(self) -> dict:
         """Parameters for the Monte Carlo evaluation (libldpc Profile D)."""
@@ -112,8 +115,9 @@ class DMinSum(torch.nn.Module):
 
 def soft_bce(L: torch.Tensor, n_tx: int) -> torch.Tensor:
     """Binary cross-entropy of the soft output against the all-zero
-    codeword's transmitted bits: mean softplus(-L) over positions < n_tx."""
-    return torch.nn.functional.softplus(-L[:, :n_tx]).mean()
+    codeword's transmitted bits: mean softplus(-L) over positions < n_tx
+    (and over iterations for a [T, B, N] output: the multi-loss)."""
+    return torch.nn.functional.softplus(-L[..., :n_tx]).mean()
 
 
 def channel_llr(code, ebn0_db: torch.Tensor, generator: Optional[torch.Generator] = None,
@@ -140,11 +144,14 @@ class TrainLog:
 
 def train(model: DMinSum, steps: int, batch: int = 65536, micro_batch: int = 16384, lr: float = 1e-3,
           ebn0_range=(1.0, 2.5), seed: int = 0, bf16_llr: bool = False, log_every: int = 10,
-          callback=None) -> List[TrainLog]:
-    """Adam on the soft BCE; every step draws `batch` codewords at Eb/N0
-    uniform in `ebn0_range`, processed in micro-batches (gradients summed)."""
+          callback=None, opt: Optional[torch.optim.Optimizer] = None, multi_loss: bool = False) -> List[TrainLog]:
+    """Adam on the soft BCE (of the last iteration, or averaged over all
+    iterations with `multi_loss`); every step draws `batch` codewords at
+    Eb/N0 uniform in `ebn0_range`, processed in micro-batches (gradients
+    summed).  Pass `opt` to continue with an existing optimizer's state."""
     dev = model.alpha.device
-    opt = torch.optim.Adam(model.parameters(), lr=lr)
+    if opt is None:
+        opt = torch.optim.Adam(model.parameters(), lr=lr)
     gen = torch.Generator(device=dev)
     gen.manual_seed(seed)
     log: List[TrainLog] = []
@@ -158,7 +165,7 @@ def train(model: DMinSum, steps: int, batch: int = 65536, micro_batch: int = 163
             b = min(micro_batch, batch - m * micro_batch)
             eb = ebn0_range[0] + (ebn0_range[1] - ebn0_range[0]) * torch.rand(b, device=dev, generator=gen)
             llr = channel_llr(model.code, eb, gen, bf16_llr)
-            loss = soft_bce(model(llr), model.n_tx) * (b / batch)
+            loss = soft_bce(model(llr, all_iters=multi_loss), model.n_tx) * (b / batch)
             loss.backward()
             total += float(loss.detach())
         opt.step()

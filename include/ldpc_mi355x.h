@@ -57,7 +57,14 @@ enum {
     LDPC_PROFILE_R = 1,
     /* Same engine built with USE_NORMALIZED_MIN_SUM: min-sum, alpha 0.75
      * (ldpc_decoder_core.h:320-411; the engine's -n flag). */
-    LDPC_PROFILE_R_NMS = 2
+    LDPC_PROFILE_R_NMS = 2,
+    /* Learned layered min-sum (no reference counterpart): the forward of the
+     * differentiable decoder of ldpc_mi355x.dmin with a syndrome early exit.
+     * fp32 LLRs and messages, r = copysign(min(max(alpha_p g - beta_p, 0),
+     * r_max), sign) with g the leave-one-out minimum and alpha / beta per
+     * (layer, circulant) from ldpc_set_dmin_params; channel as Profile T.
+     * AR4JA codes with z <= 1024 (all but K=16384 r1/2). */
+    LDPC_PROFILE_D = 3
 };
 
 /* Code families. */
@@ -248,6 +255,15 @@ int ldpc_batch_code_info(const ldpc_batch *b, uint32_t i, ldpc_code_info *info);
 const char *ldpc_batch_last_error(const ldpc_batch *b);
 
 void ldpc_batch_destroy(ldpc_batch *b);
+
+/* Profile D parameters: alpha and beta are [groups][n_params] fp32 (row t
+ * for iteration t, the last row beyond; groups = 1 shares one row), n_params
+ * = the sum of the layer degrees (n_edges / z); correction != 0 applies the
+ * Approx-Min* correction to the min-sum magnitude; r_max > 0 caps message
+ * magnitudes.  Copied to the device; required before Profile D work.  No
+ * reference counterpart (the trained decoder's parameters). */
+int ldpc_set_dmin_params(ldpc_ctx *ctx, const float *alpha, const float *beta, uint32_t groups, uint32_t n_params,
+                         int correction, float r_max);
 
 /* Wait for all work on the context's stream. */
 int ldpc_sync(ldpc_ctx *ctx);

@@ -128,3 +128,60 @@ def fd_grad(code, llr, alpha, beta, T, which, indices, eps=1e-3, corr=False, r_m
             vals.append(loss(forward(code, llr, a, b, T, corr, np.float64, r_max), code.n_tx))
         out.append((vals[0] - vals[1]) / (2 * eps))
     return np.array(out)
+
+
+def _syndrome_fail(L, sch):
+    """Per codeword: some check has odd parity of hard decisions (L < 0)."""
+    hard = L < 0
+    fail = np.zeros(L.shape[0], dtype=bool)
+    for deg, z, var, _, _ in sch:
+        par = np.bitwise_xor.reduce(hard[:, var], axis=1)
+        fail |= par.any(axis=1)
+    return fail
+
+
+def decode_d(code, llr_bf16, alpha, beta, max_iter, corr=False, r_max=0.0, return_float=False):
+    """Profile D (learned min-sum, float32) with the syndrome early exit:
+    bf16 channel LLRs in; returns (final L as RNE bf16 bits, iterations,
+    converged[, final float32 L]).  Row t of alpha / beta is used at
+    iteration t (the last row beyond); converged codewords are frozen."""
+    dt = np.float32
+    sch = schedule(code)
+    E = sum(l[0] * l[1] for l in sch)
+    llr = (np.asarray(llr_bf16, np.uint16).astype(np.uint32) << 16).view(np.float32).reshape(-1, code.n)
+    B = llr.shape[0]
+    L = llr.copy()
+    R = np.zeros((B, E), dt)
+    alpha = np.asarray(alpha, dt).reshape(-1, sum(l[0] for l in sch))
+    beta = np.asarray(beta, dt).reshape(alpha.shape)
+    iters = np.full(B, max_iter, np.uint32)
+    ok = np.zeros(B, np.uint8)
+    active = np.arange(B)
+    for t in range(max_iter):
+        if active.size == 0:
+            break
+        g = min(t, alpha.shape[0] - 1)
+        La, Ra = L[active], R[active]
+        for lay in sch:
+            _layer(La, Ra, lay, alpha[g], beta[g], corr, dt, r_max=r_max)
+        L[active], R[active] = La, Ra
+        conv = ~_syndrome_fail(La, sch)
+        iters[active[conv]] = t + 1
+        ok[active[conv]] = 1
+        active = active[~conv]
+    u = L.view(np.uint32)
+    out = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint16)
+    return (out, iters, ok, L) if return_float else (out, iters, ok)
+
+
+def run_point_d(code, llr_bf16, alpha, beta, max_iter, corr=False, r_max=0.0):
+    """Monte Carlo tallies of Profile D on given channel LLRs: dict(blocks,
+    frame_errors, bit_errors, iter_sum, hist) as the Profile T tallies (bit
+    errors = information bits <= 0 of failed codewords, from the float32 L)."""
+    _, iters, ok, L = decode_d(code, llr_bf16, alpha, beta, max_iter, corr, r_max, return_float=True)
+    hist = np.zeros(max_iter + 1, np.uint64)
+    for it, o in zip(iters, ok):
+        hist[it if o else 0] += 1
+    fail = ok == 0
+    be = int((L[fail, :code.k] <= 0).sum())
+    return dict(blocks=len(ok), frame_errors=int(fail.sum()), bit_errors=be, iter_sum=int(iters.sum()), hist=hist)

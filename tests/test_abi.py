@@ -161,3 +161,17 @@ def test_missing_library_fails_loudly(monkeypatch, tmp_path):
     monkeypatch.setattr(L, "LIB", str(tmp_path / "nope.so"))
     with pytest.raises(ImportError):
         L.load()
+
+
+def test_profile_d_arguments_checked_without_gpu():
+    lib = L.load()
+    h = C.c_void_p()
+    big = L.CodeDesc(L.LDPC_CODE_AR4JA, 16384, 1, 2)
+    cfg = L.DecoderCfg(L.LDPC_PROFILE_D, 20, 0.0)
+    assert lib.ldpc_create(C.byref(big), C.byref(cfg), 0, C.byref(h)) == L.LDPC_ERR_UNSUPPORTED
+    code = L.CodeDesc(L.LDPC_CODE_AR4JA, 1024, 1, 2)
+    cfg_v = L.DecoderCfg(L.LDPC_PROFILE_D, 20, 0.0, 0, 0, 0, 0, L.variant(1, 0))
+    assert lib.ldpc_create(C.byref(code), C.byref(cfg_v), 0, C.byref(h)) == L.LDPC_ERR_ARG
+    mat = L.CodeDesc(L.LDPC_CODE_MATRIX, 0, 0, 0, 10, 5, 0, 0, None, None)
+    assert lib.ldpc_create(C.byref(mat), C.byref(cfg), 0, C.byref(h)) == L.LDPC_ERR_UNSUPPORTED
+    assert lib.ldpc_set_dmin_params(None, None, None, 1, 60, 0, 0.0) == L.LDPC_ERR_ARG

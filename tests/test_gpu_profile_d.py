@@ -1,0 +1,66 @@
+"""Profile D (the learned layered min-sum with a syndrome early exit,
+decode_multi.hip) against the oracle, bit for bit: final LLRs (bf16),
+iterations and convergence of channel dumps; Monte Carlo counters and
+histograms; the channel is the Profile T stream."""
+import numpy as np
+import pytest
+
+import dmin_oracle as DO
+from conftest import channel_params
+from ldpc_mi355x import codes, decoder
+
+pytestmark = pytest.mark.gpu
+
+CASES = [(1024, "1/2", 1.6), (1024, "4/5", 3.0), (4096, "2/3", 2.0), (1024, "2/3", 2.4)]
+
+
+def params(code, G, seed):
+    rng = np.random.default_rng(seed)
+    P = DO.n_params(code)
+    return rng.uniform(0.7, 0.95, (G, P)).astype(np.float32), rng.uniform(0.0, 0.4, (G, P)).astype(np.float32)
+
+
+@pytest.mark.parametrize("k,rate,ebn0", CASES)
+@pytest.mark.parametrize("G,corr,r_max", [(1, False, 0.0), (8, True, 0.0), (1, False, 6.0)])
+def test_decode_matches_oracle(oracle, k, rate, ebn0, G, corr, r_max):
+    code = codes.ar4ja(k, rate)
+    mu, sg = channel_params(ebn0, code.k / code.n_tx)
+    n = {1024: 128, 4096: 24}[k]
+    llr = oracle.gen_channel(code, 5, 0, n, mu, sg)
+    a, b = params(code, G, k + G)
+    print(f"Profile D decode K={k} rate={rate} G={G} corr={corr} r_max={r_max}", flush=True)
+    with decoder.Engine(k, rate, max_iter=25, profile="D") as eng:
+        eng.set_dmin_params(a, b, corr, r_max)
+        out, it, ok = eng.decode_llrs(llr)
+    e_out, e_it, e_ok = DO.decode_d(code, llr, a, b, 25, corr, r_max)
+    np.testing.assert_array_equal(ok, e_ok)
+    np.testing.assert_array_equal(it, e_it)
+    np.testing.assert_array_equal(out, e_out)
+
+
+@pytest.mark.parametrize("k,rate,ebn0", CASES[:2])
+def test_monte_carlo_matches_oracle(oracle, k, rate, ebn0):
+    code = codes.ar4ja(k, rate)
+    mu, sg = channel_params(ebn0, code.k / code.n_tx)
+    n = 1536
+    a, b = params(code, 1, 3)
+    with decoder.Engine(k, rate, max_iter=30, profile="D") as eng:
+        eng.set_dmin_params(a, b)
+        got = eng.run_point(ebn0, n, seed=17, first_cw=9, chunk_cw=n)
+    llr = oracle.gen_channel(code, 17, 9, n, mu, sg)
+    exp = DO.run_point_d(code, llr, a, b, 30)
+    assert (got.blocks, got.frame_errors, got.bit_errors, got.iter_sum) == \
+        (exp["blocks"], exp["frame_errors"], exp["bit_errors"], exp["iter_sum"])
+    np.testing.assert_array_equal(got.hist, exp["hist"])
+
+
+def test_same_channel_as_profile_t():
+    with decoder.Engine(1024, "1/2", profile="D", max_iter=20) as d, decoder.Engine(1024, "1/2") as t:
+        np.testing.assert_array_equal(d.gen_channel(3, 100, 4, 1.5), t.gen_channel(3, 100, 4, 1.5))
+
+
+def test_parameters_required():
+    from ldpc_mi355x import _lib as L
+    with decoder.Engine(1024, "1/2", profile="D", max_iter=20) as eng:
+        with pytest.raises(L.LdpcError):
+            eng.run_point(2.0, 64)

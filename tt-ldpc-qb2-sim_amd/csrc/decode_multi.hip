@@ -529,10 +529,368 @@ hipError_t attributes(hipFuncAttributes *a) {
     return hipFuncGetAttributes(a, reinterpret_cast<const void *>(kernel_of<CF>()));
 }
 
+// ---------------------------------------------------------------------------
+// Profile D: the learned layered min-sum decoder (the forward of the
+// differentiable decoder, csrc/dmin_torch.hip) with a syndrome early exit.
+// fp32 L in LDS, fp32 R in VGPRs, one check per thread; the channel is the
+// Profile T stream (bf16 LLRs), so both profiles see the same noise.
+//   q = L - R; (min1, idx1), (min2, idx2) first-index; g = min2 on idx1,
+//   min1 (or max(min1 - corr(min2 - min1), 0)) elsewhere; z = alpha g - beta;
+//   r = copysign(min(max(z, 0), r_max), s ^ sign q); L = q + r; R = r.
+// Hard decision L < 0, bit errors L <= 0, as in Profile T.
+// ---------------------------------------------------------------------------
+template <int ORD_>
+struct CfgD {
+    static constexpr int C = 1, ORD = ORD_;
+    static constexpr ar4ja::Schedule S = ar4ja::make_schedule(LDPC_KID, LDPC_RID, ORD_);
+    static constexpr int Z = S.Z, N = S.N, NP = S.n_punct, NTX = S.N - S.n_punct, K = S.k_info;
+    static constexpr int G = Z;
+    static constexpr bool WAVE = G <= 64;
+    static constexpr int SLOTS = WAVE ? 64 / G : 1;
+    static constexpr int WPW = WAVE ? 4 : G / 64;
+    static constexpr int THREADS = 64 * WPW;
+    static constexpr int CW_PER_WG = WAVE ? SLOTS * WPW : 1;
+    static constexpr int r_off(int li) {
+        int o = 0;
+        for (int i = 0; i < li; ++i) o += S.layer[i].deg;
+        return o;
+    }
+    static constexpr int NR = r_off(NLAYERS);  // messages per thread = parameters per row
+    static_assert(G >= 16 && G <= 1024, "one check per thread");
+};
+
+template <class CF, int LI>
+__device__ __forceinline__ void run_layer_d(float *Ls, float (&R)[CF::NR], int t, const float *al, const float *be,
+                                            bool corr, float r_max) {
+    constexpr ar4ja::Layer Ly = CF::S.layer[LI];
+    constexpr int D = Ly.deg, R0 = CF::r_off(LI), Z = CF::Z;
+    asm volatile("" : "+v"(t));
+    int v[D];
+    float q[D];
+    float m1 = __builtin_inff(), m2 = __builtin_inff();
+    int i1 = 0, i2 = 0;
+    uint32_t s = 0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        v[d] = Ly.slot[d].base + ((t + Ly.slot[d].shift) & (Z - 1));
+        q[d] = __fsub_rn(Ls[v[d]], R[R0 + d]);
+        s ^= f2u(q[d]) >> 31;
+        const float a = fabsf(q[d]);
+        if (a < m1) {
+            m2 = m1;
+            i2 = i1;
+            m1 = a;
+            i1 = d;
+        } else if (a < m2) {
+            m2 = a;
+            i2 = d;
+        }
+    }
+    (void)i2;
+    float other = m1;
+    if (corr) {
+        const float C1 = 0.69140625f, K1 = -0.3984375f, C2 = 0.65625f, K2 = -0.326171875f;
+        const float x = __fsub_rn(m2, m1);
+        const float l1 = __fadd_rn(__fmul_rn(K1, x), C1);
+        const float l2 = __fadd_rn(__fmul_rn(K2, x), C2);
+        const float lm = l1 >= l2 ? l1 : l2;
+        const float o = __fsub_rn(m1, lm > 0.0f ? lm : 0.0f);
+        other = o > 0.0f ? o : 0.0f;
+    }
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        const float g = d == i1 ? m2 : other;
+        const float z = __fsub_rn(__fmul_rn(al[R0 + d], g), be[R0 + d]);
+        float mag = z > 0.0f ? z : 0.0f;
+        if (r_max > 0.0f && !(mag < r_max)) mag = r_max;
+        const uint32_t sb = (s ^ (f2u(q[d]) >> 31)) & 1u;
+        const float r = u2f(f2u(mag) | (sb << 31));
+        Ls[v[d]] = __fadd_rn(q[d], r);
+        R[R0 + d] = r;
+    }
+}
+
+template <class CF, int... LI>
+__device__ __forceinline__ void run_iteration_d(float *Ls, float (&R)[CF::NR], int t, const float *al, const float *be,
+                                                bool corr, float r_max, std::integer_sequence<int, LI...>) {
+    if constexpr (CF::WAVE)
+        ((run_layer_d<CF, LI>(Ls, R, t, al, be, corr, r_max), wave_fence()), ...);
+    else
+        ((run_layer_d<CF, LI>(Ls, R, t, al, be, corr, r_max), __syncthreads()), ...);
+}
+
+template <class CF, int LI>
+__device__ __forceinline__ bool layer_fails_d(const float *Ls, int t) {
+    constexpr ar4ja::Layer Ly = CF::S.layer[LI];
+    asm volatile("" : "+v"(t));
+    uint32_t p = 0;
+#pragma unroll
+    for (int d = 0; d < Ly.deg; ++d) p ^= Ls[Ly.slot[d].base + ((t + Ly.slot[d].shift) & (CF::Z - 1))] < 0.0f;
+    return p != 0u;
+}
+
+template <class CF, int LI>
+__device__ __forceinline__ uint32_t syndrome_pass_d(const float *Ls, int t, uint32_t pass) {
+    if constexpr (LI == NLAYERS) {
+        return pass;
+    } else {
+        asm volatile("" ::: "memory");
+        pass &= ~slot_any<CF>(__ballot(layer_fails_d<CF, LI>(Ls, t)));
+        if (pass == 0) return 0;
+        return syndrome_pass_d<CF, LI + 1>(Ls, t, pass);
+    }
+}
+
+template <class CF, int LI>
+__device__ __forceinline__ bool wave_fails_from_d(const float *Ls, int t) {
+    if constexpr (LI == NLAYERS) {
+        return false;
+    } else {
+        asm volatile("" ::: "memory");
+        if (__ballot(layer_fails_d<CF, LI>(Ls, t)) != 0) return true;
+        return wave_fails_from_d<CF, LI + 1>(Ls, t);
+    }
+}
+
+// Profile T channel (bf16 LLRs) as fp32 values.
+template <class CF>
+__device__ __forceinline__ void gen_channel_d(float *Ls, int t, uint64_t seed, uint64_t w, float mu, float sigma) {
+    const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
+    for (int b = t; b < CF::NTX / 4; b += CF::G) {
+        const u32x4 x = philox4x32_10((uint32_t)w, (uint32_t)(w >> 32), (uint32_t)b, 0u, k0, k1);
+        float z0, z1, z2, z3;
+        box_muller(x.x, x.y, z0, z1);
+        box_muller(x.z, x.w, z2, z3);
+        *reinterpret_cast<float4 *>(Ls + 4 * b) =
+            make_float4(u2f(llr_bits(mu, sigma, z0)), u2f(llr_bits(mu, sigma, z1)), u2f(llr_bits(mu, sigma, z2)),
+                        u2f(llr_bits(mu, sigma, z3)));
+    }
+    for (int b = t; b < CF::NP / 4; b += CF::G)
+        *reinterpret_cast<float4 *>(Ls + CF::NTX + 4 * b) = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
+template <class CF>
+__device__ __forceinline__ void copy_in_d(float *Ls, const uint16_t *src, int t) {
+    for (int i = t; i < CF::N; i += CF::G) Ls[i] = u2f((uint32_t)src[i] << 16);
+}
+
+template <class CF>
+__device__ __forceinline__ void copy_out_d(uint16_t *dst, const float *Ls, int t) {
+    for (int i = t; i < CF::N; i += CF::G) dst[i] = (uint16_t)(rne_bits(Ls[i]) >> 16);
+}
+
+template <class CF>
+__device__ __forceinline__ uint32_t info_errors_d(const float *Ls, int t) {
+    uint32_t be = 0;
+    for (int i = t; i < CF::K; i += CF::G) be += Ls[i] <= 0.0f ? 1u : 0u;
+    return be;
+}
+
+__device__ __forceinline__ int d_row(const MultiParams &P, int it) {
+    return P.d_groups <= 1 ? 0 : (it < P.d_groups ? it : P.d_groups - 1);
+}
+
+template <class CF>
+__global__ void __launch_bounds__(CF::THREADS) k_d_wave(MultiParams P) {
+    __shared__ __attribute__((aligned(16))) float smem[CF::CW_PER_WG * CF::N];
+    __shared__ uint32_t hist_s[HIST_LDS];
+    const Run run = load_run(P);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int slot = lane / CF::G, t = lane % CF::G;
+    float *Ls = smem + (wv * CF::SLOTS + slot) * CF::N;
+    for (int i = threadIdx.x; i < HIST_LDS; i += CF::THREADS) hist_s[i] = 0u;
+    __syncthreads();
+    const uint32_t n_slots = gridDim.x * CF::WPW * CF::SLOTS;
+    const uint32_t gslot = (blockIdx.x * CF::WPW + wv) * CF::SLOTS + slot;
+    const uint32_t n_static = (uint32_t)(((uint64_t)run.n_cw * 7u / 8u) / n_slots * n_slots);
+    const uint32_t all_slots = (1u << CF::SLOTS) - 1u;
+    const bool corr = P.d_corr != 0;
+    uint32_t kstat = 0, cw = run.n_cw;
+    int it = 0;
+    float R[CF::NR];
+#pragma unroll
+    for (int i = 0; i < CF::NR; ++i) R[i] = 0.0f;
+    unsigned long long blocks = 0, fe = 0, be = 0, its = 0;
+    uint32_t done = all_slots, conv = 0, active = 0;
+    for (;;) {
+        if (done) {
+            const bool mine = (done >> slot) & 1u;
+            if (mine && cw < run.n_cw) {
+                const bool ok = (conv >> slot) & 1u;
+                if (!ok) be += info_errors_d<CF>(Ls, t);
+                if (P.llr_in) copy_out_d<CF>(P.llr_out + (size_t)cw * CF::N, Ls, t);
+                if (t == 0) {
+                    blocks += 1;
+                    its += ok ? (unsigned)it : (unsigned)P.max_iter;
+                    fe += ok ? 0u : 1u;
+                    hist_add(hist_s, P, ok ? it : 0);
+                    if (P.llr_in) {
+                        P.iters[cw] = ok ? (uint32_t)it : (uint32_t)P.max_iter;
+                        P.ok[cw] = ok ? 1 : 0;
+                    }
+                }
+            }
+            uint32_t nxt = run.n_cw;
+            if (mine && t == 0) {
+                const uint32_t s = gslot + kstat * n_slots;
+                if (s < n_static) {
+                    nxt = s;
+                } else {
+                    const uint32_t v = atomicAdd(P.queue, 1u);
+                    nxt = v >= run.n_cw - n_static ? run.n_cw : n_static + v;
+                }
+            }
+            nxt = __shfl(nxt, slot * CF::G);
+            if (mine) {
+                kstat += 1;
+                cw = nxt;
+                it = 0;
+                if (cw < run.n_cw) {
+                    if (P.llr_in)
+                        copy_in_d<CF>(Ls, P.llr_in + (size_t)cw * CF::N, t);
+                    else
+                        gen_channel_d<CF>(Ls, t, run.seed, run.first_cw + cw, run.mu, run.sigma);
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < CF::NR; ++i) R[i] = mine ? 0.0f : R[i];
+            active = slot_any<CF>(__ballot(cw < run.n_cw));
+            wave_fence();
+        }
+        if (!active) break;
+        const int row = d_row(P, it);
+        run_iteration_d<CF>(Ls, R, t, P.d_alpha + row * CF::NR, P.d_beta + row * CF::NR, corr, P.d_rmax,
+                            std::make_integer_sequence<int, NLAYERS>{});
+        it += 1;
+        conv = syndrome_pass_d<CF, 0>(Ls, t, active);
+        done = (conv | slot_any<CF>(__ballot(it >= P.max_iter))) & active;
+    }
+    blocks = wave_sum(blocks);
+    fe = wave_sum(fe);
+    be = wave_sum(be);
+    its = wave_sum(its);
+    if (lane == 0) flush_counters(P, blocks, fe, be, its);
+    __syncthreads();
+    hist_flush(hist_s, P, CF::THREADS);
+}
+
+template <class CF>
+__global__ void __launch_bounds__(CF::THREADS) k_d_group(MultiParams P) {
+    __shared__ __attribute__((aligned(16))) float Ls[CF::N];
+    __shared__ uint32_t hist_s[HIST_LDS];
+    __shared__ uint32_t flag_s[CF::WPW];
+    __shared__ uint32_t cw_s, be_s;
+    const Run run = load_run(P);
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    for (int i = t; i < HIST_LDS; i += CF::THREADS) hist_s[i] = 0u;
+    const uint32_t n_wg = gridDim.x;
+    const uint32_t n_static = (uint32_t)(((uint64_t)run.n_cw * 7u / 8u) / n_wg * n_wg);
+    const bool corr = P.d_corr != 0;
+    uint32_t kstat = 0;
+    unsigned long long blocks = 0, fe = 0, be = 0, its = 0;
+    float R[CF::NR];
+    __syncthreads();
+    for (;;) {
+        uint32_t cw;
+        const uint32_t s = blockIdx.x + kstat * n_wg;
+        if (s < n_static) {
+            cw = s;
+            ++kstat;
+        } else {
+            if (t == 0) {
+                const uint32_t v = atomicAdd(P.queue, 1u);
+                cw_s = v >= run.n_cw - n_static ? run.n_cw : n_static + v;
+            }
+            __syncthreads();
+            cw = cw_s;
+        }
+        if (cw >= run.n_cw) break;
+        if (P.llr_in)
+            copy_in_d<CF>(Ls, P.llr_in + (size_t)cw * CF::N, t);
+        else
+            gen_channel_d<CF>(Ls, t, run.seed, run.first_cw + cw, run.mu, run.sigma);
+        if (t == 0) be_s = 0u;
+#pragma unroll
+        for (int i = 0; i < CF::NR; ++i) R[i] = 0.0f;
+        __syncthreads();
+        int it = 0;
+        bool ok = false;
+        while (it < P.max_iter) {
+            const int row = d_row(P, it);
+            run_iteration_d<CF>(Ls, R, t, P.d_alpha + row * CF::NR, P.d_beta + row * CF::NR, corr, P.d_rmax,
+                                std::make_integer_sequence<int, NLAYERS>{});
+            ++it;
+            const bool f = wave_fails_from_d<CF, 0>(Ls, t);
+            if (lane == 0) flag_s[wv] = f ? 1u : 0u;
+            __syncthreads();
+            uint32_t any = 0;
+#pragma unroll
+            for (int w = 0; w < CF::WPW; ++w) any |= flag_s[w];
+            if (!any) {
+                ok = true;
+                break;
+            }
+        }
+        if (!ok) {
+            const uint32_t e = info_errors_d<CF>(Ls, t);
+            if (e) atomicAdd(&be_s, e);
+        }
+        if (P.llr_in) copy_out_d<CF>(P.llr_out + (size_t)cw * CF::N, Ls, t);
+        __syncthreads();
+        if (t == 0) {
+            blocks += 1;
+            its += ok ? (unsigned)it : (unsigned)P.max_iter;
+            if (!ok) {
+                fe += 1;
+                be += be_s;
+            }
+            hist_add(hist_s, P, ok ? it : 0);
+            if (P.llr_in) {
+                P.iters[cw] = ok ? (uint32_t)it : (uint32_t)P.max_iter;
+                P.ok[cw] = ok ? 1 : 0;
+            }
+        }
+        __syncthreads();
+    }
+    if (t == 0) flush_counters(P, blocks, fe, be, its);
+    __syncthreads();
+    hist_flush(hist_s, P, CF::THREADS);
+}
+
+template <class CF>
+constexpr auto kernel_of_d() {
+    if constexpr (CF::WAVE)
+        return &k_d_wave<CF>;
+    else
+        return &k_d_group<CF>;
+}
+
+template <class CF>
+hipError_t launch_d(const MultiParams &p, int grid, hipStream_t st) {
+    hipLaunchKernelGGL(kernel_of_d<CF>(), dim3(grid), dim3(CF::THREADS), 0, st, p);
+    return hipGetLastError();
+}
+
+template <class CF>
+hipError_t occupancy_d(int *wg_per_cu) {
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(wg_per_cu, kernel_of_d<CF>(), CF::THREADS, 0);
+}
+
+template <class CF>
+hipError_t attributes_d(hipFuncAttributes *a) {
+    return hipFuncGetAttributes(a, reinterpret_cast<const void *>(kernel_of_d<CF>()));
+}
+
 template <class CF>
 constexpr MultiVariant variant() {
     return MultiVariant{LDPC_KID, LDPC_RID, CF::C, CF::ORD, CF::THREADS, CF::CW_PER_WG,
-                        &launch<CF>, &occupancy<CF>, &attributes<CF>};
+                        &launch<CF>, &occupancy<CF>, &attributes<CF>, 0};
+}
+
+template <class CF>
+constexpr MultiVariant variant_d() {
+    return MultiVariant{LDPC_KID, LDPC_RID, CF::C, CF::ORD, CF::THREADS, CF::CW_PER_WG,
+                        &launch_d<CF>, &occupancy_d<CF>, &attributes_d<CF>, 1};
 }
 
 // C = 1 needs Z <= 1024 threads per codeword (all codes but K = 16384 r1/2).
@@ -546,16 +904,32 @@ const MultiVariant TABLE[] = {
     variant<Cfg<1, 1>>(),
 #endif
 };
+
+// Profile D (one check per thread: not for K = 16384 r1/2, z = 2048)
+#if !(LDPC_KID == 2 && LDPC_RID == 0)
+const MultiVariant TABLE_D[] = {variant_d<CfgD<0>>()};
+#else
+const MultiVariant *const TABLE_D = nullptr;
+#endif
 static_assert(C1_OK == !(LDPC_KID == 2 && LDPC_RID == 0), "C = 1 availability");
 
 }  // namespace
 
 #define LDPC_CAT3(a, b, c, d) a##b##c##d
 #define LDPC_TABLE_NAME(k, r) LDPC_CAT3(multi_table_k, k, _r, r)
+#define LDPC_TABLE_D_NAME(k, r) LDPC_CAT3(multi_table_d_k, k, _r, r)
 
 const MultiVariant *LDPC_TABLE_NAME(LDPC_KID, LDPC_RID)(int *n) {
     *n = (int)(sizeof(TABLE) / sizeof(TABLE[0]));
     return TABLE;
+}
+
+const MultiVariant *LDPC_TABLE_D_NAME(LDPC_KID, LDPC_RID)() {
+#if !(LDPC_KID == 2 && LDPC_RID == 0)
+    return &TABLE_D[0];
+#else
+    return nullptr;
+#endif
 }
 
 }  // namespace ldpc

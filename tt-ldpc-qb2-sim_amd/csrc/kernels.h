@@ -60,6 +60,11 @@ struct MultiParams {
     uint32_t n_cw;
     int max_iter;
     float mu, sigma, beta, beta3;
+    // Profile D (learned min-sum): scale / offset rows [n_groups][P], P = sum
+    // of the layer degrees; iteration t uses row min(t, n_groups - 1)
+    const float *d_alpha, *d_beta;
+    int d_groups, d_corr;
+    float d_rmax;
 };
 struct MultiVariant {
     int kid, rid, c, order;  // code (K = 1024 << 2 kid, rate 1/2, 2/3, 4/5), checks per thread, layer order
@@ -67,6 +72,7 @@ struct MultiVariant {
     hipError_t (*launch)(const MultiParams &p, int grid, hipStream_t st);
     hipError_t (*occupancy)(int *wg_per_cu);
     hipError_t (*attributes)(hipFuncAttributes *a);
+    int profile_d;  // 1: the learned min-sum kernel (Profile D), 0: Profile T
 };
 const MultiVariant *multi_table_k0_r0(int *n);
 const MultiVariant *multi_table_k0_r1(int *n);
@@ -77,6 +83,16 @@ const MultiVariant *multi_table_k1_r2(int *n);
 const MultiVariant *multi_table_k2_r0(int *n);
 const MultiVariant *multi_table_k2_r1(int *n);
 const MultiVariant *multi_table_k2_r2(int *n);
+// Profile D kernel of a code (null where not built: K = 16384 r1/2)
+const MultiVariant *multi_table_d_k0_r0();
+const MultiVariant *multi_table_d_k0_r1();
+const MultiVariant *multi_table_d_k0_r2();
+const MultiVariant *multi_table_d_k1_r0();
+const MultiVariant *multi_table_d_k1_r1();
+const MultiVariant *multi_table_d_k1_r2();
+const MultiVariant *multi_table_d_k2_r0();
+const MultiVariant *multi_table_d_k2_r1();
+const MultiVariant *multi_table_d_k2_r2();
 
 hipError_t launch_fill_u64(unsigned long long *p, size_t n, unsigned long long v, hipStream_t st);
 hipError_t launch_fill_u32(uint32_t *p, size_t n, uint32_t v, hipStream_t st);

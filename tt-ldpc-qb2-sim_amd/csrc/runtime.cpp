@@ -39,8 +39,12 @@ struct ldpc_ctx {
     unsigned long long *d_counters = nullptr;  // [4]
     unsigned long long *d_hist = nullptr;      // [max_iter + 1]
     // Profile T on the generic nine-code kernels (null: the specialised
-    // AR4JA r1/2 K=1024 kernel)
+    // AR4JA r1/2 K=1024 kernel), or the Profile D kernel
     const ldpc::MultiVariant *mv = nullptr;
+    // Profile D parameters on the device
+    float *d_alpha = nullptr, *d_beta = nullptr;
+    int d_groups = 0, d_corr = 0;
+    float d_rmax = 0.0f;
     // Profile R: level-ordered pass table on the device
     void *d_table = nullptr;
     ldpc::RLaunch rl{};
@@ -237,6 +241,14 @@ const MultiVariant *find_variant(int k_id, int rate_id, uint32_t variant) {
 // (K = 1024 / 4096 / 16384 rows; rates 1/2, 2/3, 4/5 columns).  One check per
 // thread wins wherever it exists except K = 4096 r1/2; K = 16384 r1/2
 // (z = 2048) has two-check threads only.
+const MultiVariant *profile_d_kernel(int k_id, int rate_id) {
+    static const MultiVariant *(*const T[3][3])() = {
+        {multi_table_d_k0_r0, multi_table_d_k0_r1, multi_table_d_k0_r2},
+        {multi_table_d_k1_r0, multi_table_d_k1_r1, multi_table_d_k1_r2},
+        {multi_table_d_k2_r0, multi_table_d_k2_r1, multi_table_d_k2_r2}};
+    return T[k_id][rate_id]();
+}
+
 uint32_t default_variant(int k_id, int rate_id) {
     static const int C[3][3] = {{1, 1, 1}, {2, 1, 1}, {2, 1, 1}};
     return LDPC_VARIANT(C[k_id][rate_id], 0);
@@ -280,10 +292,18 @@ int ldpc_create(const ldpc_code_desc *code, const ldpc_decoder_cfg *cfg, int dev
     int k_id = -1, rate_id = -1;
     if (code->family == LDPC_CODE_AR4JA && !ldpc::ar4ja_ids(*code, k_id, rate_id))
         return fail(nullptr, LDPC_ERR_ARG, "no AR4JA code with this (k_info, rate)");
-    if (cfg->profile > LDPC_PROFILE_R_NMS) return fail(nullptr, LDPC_ERR_UNSUPPORTED, "unknown decoder profile");
+    if (cfg->profile > LDPC_PROFILE_D) return fail(nullptr, LDPC_ERR_UNSUPPORTED, "unknown decoder profile");
     if (cfg->max_iter < 1 || cfg->max_iter > 100000) return fail(nullptr, LDPC_ERR_ARG, "max_iter out of range");
     const bool prof_t = cfg->profile == LDPC_PROFILE_T;
-    if (prof_t) {
+    const bool prof_d = cfg->profile == LDPC_PROFILE_D;
+    if (prof_d) {
+        if (code->family != LDPC_CODE_AR4JA)
+            return fail(nullptr, LDPC_ERR_UNSUPPORTED, "Profile D kernels are built for the AR4JA codes");
+        if (!ldpc::profile_d_kernel(k_id, rate_id))
+            return fail(nullptr, LDPC_ERR_UNSUPPORTED, "Profile D needs z <= 1024 (not AR4JA K=16384 r1/2)");
+        if (cfg->variant || cfg->l_max != 0.0f || cfg->r_max != 0.0f || cfg->l_post_max != 0.0f || cfg->gamma != 0.0f)
+            return fail(nullptr, LDPC_ERR_ARG, "Profile D takes its settings from ldpc_set_dmin_params");
+    } else if (prof_t) {
         if (!(cfg->beta > 0.0f) || !std::isfinite(cfg->beta)) return fail(nullptr, LDPC_ERR_ARG, "beta must be > 0");
         if (cfg->l_max != 0.0f || cfg->r_max != 0.0f || cfg->l_post_max != 0.0f || cfg->gamma != 0.0f)
             return fail(nullptr, LDPC_ERR_ARG, "clamps belong to Profile R; the golden Profile T has none");
@@ -330,7 +350,13 @@ int ldpc_create(const ldpc_code_desc *code, const ldpc_decoder_cfg *cfg, int dev
     if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device)) != hipSuccess)
         return bail(e, "hipDeviceGetAttribute");
 
-    if (prof_t && !(is_r12_k1024(*code) && cfg->variant == 0)) {
+    if (prof_d) {
+        c->mv = ldpc::profile_d_kernel(k_id, rate_id);
+        ldpc::fill_code_info(k_id, rate_id, c->info);
+        int occ = 0;
+        if ((e = c->mv->occupancy(&occ)) != hipSuccess) return bail(e, "occupancy");
+        c->grid_mc = c->grid_dec = cus * (occ > 0 ? occ : 1);
+    } else if (prof_t && !(is_r12_k1024(*code) && cfg->variant == 0)) {
         c->mv = ldpc::find_variant(k_id, rate_id, cfg->variant ? cfg->variant : ldpc::default_variant(k_id, rate_id));
         ldpc::fill_code_info(k_id, rate_id, c->info);
         int occ = 0;
@@ -388,6 +414,8 @@ void ldpc_destroy(ldpc_ctx *c) {
     for (DevBuf *b : {&c->in, &c->out, &c->iters, &c->ok})
         if (b->p) (void)hipFree(b->p);
     if (c->d_table) (void)hipFree(c->d_table);
+    if (c->d_alpha) (void)hipFree(c->d_alpha);
+    if (c->d_beta) (void)hipFree(c->d_beta);
     if (c->d_queue) (void)hipFree(c->d_queue);
     if (c->d_counters) (void)hipFree(c->d_counters);
     if (c->d_hist) (void)hipFree(c->d_hist);
@@ -443,6 +471,33 @@ int ldpc_channel_params(const ldpc_ctx *c, double ebn0_db, float *mu, float *sig
     return LDPC_OK;
 }
 
+int ldpc_set_dmin_params(ldpc_ctx *c, const float *alpha, const float *beta, uint32_t groups, uint32_t n_params,
+                         int correction, float r_max) {
+    if (!c) return LDPC_ERR_ARG;
+    if (c->cfg.profile != LDPC_PROFILE_D) return fail(c, LDPC_ERR_ARG, "not a Profile D context");
+    if (!alpha || !beta || groups == 0 || groups > 100000) return fail(c, LDPC_ERR_ARG, "bad parameter arrays");
+    if (n_params != c->info.n_edges / c->info.z)
+        return fail(c, LDPC_ERR_ARG, "n_params must be the sum of the layer degrees (n_edges / z)");
+    if (!(r_max >= 0.0f) || !std::isfinite(r_max)) return fail(c, LDPC_ERR_ARG, "r_max must be >= 0");
+    const size_t nb = (size_t)groups * n_params * sizeof(float);
+    for (size_t i = 0; i < (size_t)groups * n_params; ++i)
+        if (!std::isfinite(alpha[i]) || !std::isfinite(beta[i])) return fail(c, LDPC_ERR_ARG, "non-finite parameter");
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (c->d_alpha) HIP_TRY(c, hipFree(c->d_alpha));
+    if (c->d_beta) HIP_TRY(c, hipFree(c->d_beta));
+    c->d_alpha = c->d_beta = nullptr;
+    c->d_groups = 0;
+    HIP_TRY(c, hipMalloc(&c->d_alpha, nb));
+    HIP_TRY(c, hipMalloc(&c->d_beta, nb));
+    HIP_TRY(c, hipMemcpy(c->d_alpha, alpha, nb, hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemcpy(c->d_beta, beta, nb, hipMemcpyHostToDevice));
+    c->d_groups = (int)groups;
+    c->d_corr = correction ? 1 : 0;
+    c->d_rmax = r_max;
+    return LDPC_OK;
+}
+
 int ldpc_sync(ldpc_ctx *c) {
     if (!c) return LDPC_ERR_ARG;
     HIP_TRY(c, hipSetDevice(c->device));
@@ -456,6 +511,11 @@ static float l_max_at(const ldpc_ctx *c, float mu) { return c->cfg.gamma > 0.0f 
 
 static ldpc::MultiParams multi_params(const ldpc_ctx *c) {
     ldpc::MultiParams p{};
+    p.d_alpha = c->d_alpha;
+    p.d_beta = c->d_beta;
+    p.d_groups = c->d_groups;
+    p.d_corr = c->d_corr;
+    p.d_rmax = c->d_rmax;
     p.queue = c->d_queue;
     p.counters = c->d_counters;
     p.hist = c->d_hist;
@@ -471,6 +531,8 @@ int ldpc_run_point(ldpc_ctx *c, const ldpc_point_cfg *pt, ldpc_point_result *res
     const uint64_t chunk = pt->chunk_cw ? pt->chunk_cw : (1ull << 20);
     // the device queue counter is 32-bit and also counts one exit ticket per wave
     if (chunk > (1ull << 31)) return fail(c, LDPC_ERR_ARG, "chunk_cw must be <= 2^31");
+    if (c->cfg.profile == LDPC_PROFILE_D && !c->d_groups)
+        return fail(c, LDPC_ERR_ARG, "Profile D: call ldpc_set_dmin_params first");
     HIP_TRY(c, hipSetDevice(c->device));
     float mu, sigma;
     ldpc_channel_params(c, pt->ebn0_db, &mu, &sigma);
@@ -535,6 +597,8 @@ int ldpc_decode_llrs_device_async(ldpc_ctx *c, const void *d_llr, uint32_t n_cw,
     if (!c) return LDPC_ERR_ARG;
     if (n_cw == 0) return LDPC_OK;
     if (!d_llr || !d_out_llr || !d_out_iters || !d_out_ok) return fail(c, LDPC_ERR_ARG, "null buffer");
+    if (c->cfg.profile == LDPC_PROFILE_D && !c->d_groups)
+        return fail(c, LDPC_ERR_ARG, "Profile D: call ldpc_set_dmin_params first");
     HIP_TRY(c, hipSetDevice(c->device));
     HIP_TRY(c, ldpc::launch_fill_u32(c->d_queue, 1, 0u, c->stream));
     if (c->mv) {

@@ -13,6 +13,7 @@ const MultiVariant *multi_table(int k_id, int rate_id, int *n);
 uint32_t variant_code(const MultiVariant &v);
 const MultiVariant *find_variant(int k_id, int rate_id, uint32_t variant);
 uint32_t default_variant(int k_id, int rate_id);
+const MultiVariant *profile_d_kernel(int k_id, int rate_id);
 void fill_code_info(int k_id, int rate_id, ldpc_code_info &info);
 // BPSK LLR mean / deviation at ebn0_db for rate k_info / n_tx.
 void channel_params(const ldpc_code_info &info, double ebn0_db, float *mu, float *sigma);

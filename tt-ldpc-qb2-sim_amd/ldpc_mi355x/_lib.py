@@ -20,6 +20,7 @@ LDPC_ERR_NODEV = -5
 LDPC_PROFILE_T = 0
 LDPC_PROFILE_R = 1
 LDPC_PROFILE_R_NMS = 2
+LDPC_PROFILE_D = 3
 LDPC_CODE_AR4JA = 1
 LDPC_CODE_MATRIX = 2
 
@@ -89,6 +90,8 @@ SIGNATURES = {
                                                 C.c_void_p]),
     "ldpc_gen_channel": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, C.c_uint32, C.c_double, C.c_void_p]),
     "ldpc_channel_params": (C.c_int, [C.c_void_p, C.c_double, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+    "ldpc_set_dmin_params": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_int,
+                                       C.c_float]),
     "ldpc_sync": (C.c_int, [C.c_void_p]),
     "ldpc_stream": (C.c_void_p, [C.c_void_p]),
 }

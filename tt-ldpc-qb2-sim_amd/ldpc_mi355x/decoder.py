@@ -104,7 +104,7 @@ def list_variants(k_info: int, rate: str):
     return [int(v) for v in buf[:n]]
 
 
-PROFILES = {"T": L.LDPC_PROFILE_T, "R": L.LDPC_PROFILE_R, "R_NMS": L.LDPC_PROFILE_R_NMS}
+PROFILES = {"T": L.LDPC_PROFILE_T, "R": L.LDPC_PROFILE_R, "R_NMS": L.LDPC_PROFILE_R_NMS, "D": L.LDPC_PROFILE_D}
 
 
 def autotune(k_info: int, rate: str, ebn0_db: float, n_cw: int = 1 << 16, reps: int = 2, max_iter: int = 200,
@@ -140,7 +140,9 @@ class Engine:
               ``variant`` = L.variant(checks_per_thread, layer_order) picks a
               kernel variant, 0 the library default);
       "R"     the scalar_5w recursive Approx-Min* that run_simulation.py drives;
-      "R_NMS" the same engine's normalized min-sum (its -n flag).
+      "R_NMS" the same engine's normalized min-sum (its -n flag);
+      "D"     the learned layered min-sum of ldpc_mi355x.dmin (set its
+              parameters with set_dmin_params).
     A Profile R engine takes either an AR4JA (k_info, rate) code in makeH row
     order or an explicit matrix: ``matrix=(rows, n)`` with rows in decoding
     (file) order, and ``n_punct`` punctured trailing variables.
@@ -200,6 +202,17 @@ class Engine:
         mu, sg = C.c_float(), C.c_float()
         self._check(self._lib.ldpc_channel_params(self._h, ebn0_db, C.byref(mu), C.byref(sg)))
         return mu.value, sg.value
+
+    def set_dmin_params(self, alpha, beta, correction: bool = False, r_max: float = 0.0) -> None:
+        """Profile D scale / offset rows [groups, n_params] (float32)."""
+        a = np.ascontiguousarray(alpha, dtype=np.float32)
+        b = np.ascontiguousarray(beta, dtype=np.float32)
+        if a.ndim == 1:
+            a, b = a[None, :], b[None, :]
+        if a.shape != b.shape:
+            raise ValueError("alpha and beta must have the same shape")
+        self._check(self._lib.ldpc_set_dmin_params(self._h, a.ctypes.data, b.ctypes.data, a.shape[0], a.shape[1],
+                                                   int(bool(correction)), float(r_max)))
 
     def schedule(self):
         order = L.variant_fields(self.variant)[1] if self.variant else 0

@@ -13,7 +13,9 @@ all-zero codeword plus AWGN from the counter-based stream.
 run_simulation.py drives) on the reference's own .chinn.out matrix (AR4JA
 r1/2 K=1024 relabelled, rows in file order), 2.0 dB, 200 max iterations --
 the configuration of the reference's published 158 Msps scalar figure
-(README.md:108).  ``--workload batch9`` measures the mixed-code extension: all nine
+(README.md:108).  ``--workload floor`` is the error-floor configuration: 1e9 codewords at
+2.5 dB in 2^20-codeword launches with u64 counters (about 30 s on one
+MI355X).  ``--workload batch9`` measures the mixed-code extension: all nine
 AR4JA codes (K = 1024, 4096, 16384 x rate 1/2, 2/3, 4/5), equal codewords per
 code, each at its own Eb/N0 in [1, 3] dB, one captured HIP graph per step
 holding every code's persistent kernel.
@@ -253,6 +255,31 @@ def bench_scalar(args) -> None:
     eng.close()
 
 
+def bench_floor(args) -> None:
+    from ldpc_mi355x import decoder
+    eng = decoder.Engine(1024, "1/2", max_iter=MAX_ITER, beta=BETA)
+    n, acc, first = args.blocks, None, 0
+    t0 = time.perf_counter()
+    while (acc.blocks if acc else 0) < n:
+        k = min(100 * CW_PER_STEP, n - (acc.blocks if acc else 0))
+        acc = eng.run_point(2.5, k, seed=0x5EED2026 ^ 0x2500, first_cw=first, chunk_cw=CW_PER_STEP, result=acc)
+        first = acc.next_cw
+        print(f"# {acc.blocks} codewords, {acc.frame_errors} frame errors, {time.perf_counter() - t0:.1f} s",
+              file=sys.stderr, flush=True)
+    wall = time.perf_counter() - t0
+    lo, hi = poisson_ci95(acc.frame_errors)
+    latest = max([i for i in range(1, MAX_ITER + 1) if acc.hist[i] > 0] or [0])
+    out = {"metric": "Gsps (channel sym/s) 1 GPU, AR4JA r1/2 K=1024 @2.5dB error floor, 1e9 codewords",
+           "value": acc.blocks * N_TX / wall / 1e9, "unit": "Gsps", "n_gpus": 1, "blocks": acc.blocks,
+           "launches": acc.launches, "frame_errors": acc.frame_errors, "bit_errors": acc.bit_errors,
+           "fer": acc.fer, "fer_ci95": [lo / acc.blocks, hi / acc.blocks], "mean_iterations": acc.mean_iterations,
+           "latest_converged_iteration": latest, "wall_s": wall, "kernel_gsps": acc.kernel_gsps,
+           "higher_is_better": True, "dtype": "bf16", "data": "synthetic (on-device Philox/Box-Muller)",
+           "reference_published_fer": 1.53e-9}
+    print(json.dumps(out), flush=True)
+    eng.close()
+
+
 def bench_batch9(args) -> None:
     from ldpc_mi355x import decoder
     codes9 = list(BATCH9_EBN0)
@@ -323,9 +350,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run device-to-device copies of MIB MiB (counter calibration)")
-    ap.add_argument("--workload", choices=["r12k1024", "scalar", "batch9"], default="r12k1024",
+    ap.add_argument("--workload", choices=["r12k1024", "scalar", "batch9", "floor"], default="r12k1024",
                     help="r12k1024: the headline AR4JA r1/2 K=1024 point; scalar: Profile R on the .chinn.out; "
-                         "batch9: the nine-code mixed batch")
+                         "batch9: the nine-code mixed batch; floor: 1e9 codewords at 2.5 dB")
+    ap.add_argument("--blocks", type=int, default=10 ** 9, help="codewords of the floor workload")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
@@ -340,6 +368,9 @@ def main():
         return
     if args.workload == "scalar":
         bench_scalar(args)
+        return
+    if args.workload == "floor":
+        bench_floor(args)
         return
     from ldpc_mi355x import decoder
     eng = decoder.Engine(1024, "1/2", max_iter=MAX_ITER, beta=BETA, device=0)

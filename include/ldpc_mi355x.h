@@ -256,6 +256,21 @@ const char *ldpc_batch_last_error(const ldpc_batch *b);
 
 void ldpc_batch_destroy(ldpc_batch *b);
 
+/* Trapping-set capture.  After ldpc_set_capture(ctx, max_frames) every
+ * later ldpc_run_point records the codeword id of each failing codeword
+ * (up to max_frames, in completion order; 0 turns capture off and clears
+ * it).  A captured codeword is reproduced exactly from (seed, id): its
+ * channel LLRs with ldpc_gen_channel and its final state with
+ * ldpc_decode_llrs.  Replaces the circulant engine's first-failure trap and
+ * vnode dumps (engines/circulant_dma_4cw/writer_ncrisc.cpp:330-398;
+ * ldpc_sim_circulant_dma.cpp:435-472). */
+int ldpc_set_capture(ldpc_ctx *ctx, uint32_t max_frames);
+
+/* Captured ids so far, ascending (at most cap written); *n_failed (may be
+ * NULL) receives the number of failures seen since ldpc_set_capture, which
+ * can exceed what was kept.  Returns the number written. */
+int ldpc_get_captures(ldpc_ctx *ctx, uint64_t *cw_ids, uint32_t cap, uint64_t *n_failed);
+
 /* Profile D parameters: alpha and beta are [groups][n_params] fp32 (row t
  * for iteration t, the last row beyond; groups = 1 shares one row), n_params
  * = the sum of the layer degrees (n_edges / z); correction != 0 applies the

@@ -374,6 +374,7 @@ __global__ void __launch_bounds__(CF::THREADS) k_multi_wave(MultiParams P) {
                     its += ok ? (unsigned)it : (unsigned)P.max_iter;
                     fe += ok ? 0u : 1u;
                     hist_add(hist_s, P, ok ? it : 0);
+                    if (!ok && !P.llr_in) capture_failure(P.cap, run.first_cw + cw);
                     if (P.llr_in) {
                         P.iters[cw] = ok ? (uint32_t)it : (uint32_t)P.max_iter;
                         P.ok[cw] = ok ? 1 : 0;
@@ -493,6 +494,7 @@ __global__ void __launch_bounds__(CF::THREADS) k_multi_group(MultiParams P) {
                 be += be_s;
             }
             hist_add(hist_s, P, ok ? it : 0);
+            if (!ok && !P.llr_in) capture_failure(P.cap, run.first_cw + cw);
             if (P.llr_in) {
                 P.iters[cw] = ok ? (uint32_t)it : (uint32_t)P.max_iter;
                 P.ok[cw] = ok ? 1 : 0;
@@ -724,6 +726,7 @@ __global__ void __launch_bounds__(CF::THREADS) k_d_wave(MultiParams P) {
                     its += ok ? (unsigned)it : (unsigned)P.max_iter;
                     fe += ok ? 0u : 1u;
                     hist_add(hist_s, P, ok ? it : 0);
+                    if (!ok && !P.llr_in) capture_failure(P.cap, run.first_cw + cw);
                     if (P.llr_in) {
                         P.iters[cw] = ok ? (uint32_t)it : (uint32_t)P.max_iter;
                         P.ok[cw] = ok ? 1 : 0;
@@ -845,6 +848,7 @@ __global__ void __launch_bounds__(CF::THREADS) k_d_group(MultiParams P) {
                 be += be_s;
             }
             hist_add(hist_s, P, ok ? it : 0);
+            if (!ok && !P.llr_in) capture_failure(P.cap, run.first_cw + cw);
             if (P.llr_in) {
                 P.iters[cw] = ok ? (uint32_t)it : (uint32_t)P.max_iter;
                 P.ok[cw] = ok ? 1 : 0;

@@ -273,7 +273,7 @@ __global__ void __launch_bounds__(64 * WAVES)
 template <int MAXD, bool NMS>
 __global__ void __launch_bounds__(64 * WAVES)
     k_monte_carlo_r(RArgs a, uint64_t seed, uint64_t first_cw, uint32_t n_cw, float mu, float sigma, uint32_t *queue,
-                    unsigned long long *counters, unsigned long long *hist) {
+                    unsigned long long *counters, unsigned long long *hist, Capture cap) {
     extern __shared__ __attribute__((aligned(16))) uint16_t smem_r[];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     uint16_t *Ls = smem_r + wv * wave_lds_halfs(a.n, a.n_edges);
@@ -295,7 +295,10 @@ __global__ void __launch_bounds__(64 * WAVES)
         its += it ? (uint64_t)it : (uint64_t)a.max_iter;
         be += b;
         fe += b > 0;
-        if (lane == 0) atomicAdd(hist + it, 1ull);
+        if (lane == 0) {
+            atomicAdd(hist + it, 1ull);
+            if (b > 0) capture_failure(cap, first_cw + cw);
+        }
         wave_fence();
     }
     if (lane == 0 && blocks) {
@@ -322,7 +325,7 @@ hipError_t launch_mc_t(const RLaunch &L, uint64_t seed, uint64_t first_cw, uint3
     RArgs a{(const uint4 *)L.table, L.n_pass, L.n, L.n_tx, L.n_edges, L.max_iter, L.r_max, L.l_post_max, L.l_max};
     const size_t lds = (size_t)WAVES * 2 * (((L.n + L.n_edges) + 7) & ~7);
     hipLaunchKernelGGL((k_monte_carlo_r<MAXD, NMS>), dim3(L.grid), dim3(64 * WAVES), lds, st, a, seed, first_cw, n_cw,
-                       mu, sigma, L.queue, counters, hist);
+                       mu, sigma, L.queue, counters, hist, L.cap);
     return hipGetLastError();
 }
 

@@ -311,7 +311,7 @@ constexpr int HIST_LDS = 256;
 
 __global__ void __launch_bounds__(64 * WAVES_PER_WG)
     k_monte_carlo(uint64_t seed, uint64_t first_cw, uint32_t n_cw, float mu, float sigma, int max_iter, float beta6,
-                  float beta3, uint32_t *queue, unsigned long long *counters, unsigned long long *hist) {
+                  float beta3, uint32_t *queue, unsigned long long *counters, unsigned long long *hist, Capture cap) {
     __shared__ __attribute__((aligned(16))) uint16_t smem[WAVES_PER_WG][NV];
     __shared__ uint32_t hist_s[HIST_LDS];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -339,6 +339,7 @@ __global__ void __launch_bounds__(64 * WAVES_PER_WG)
                 atomicAdd(hist_s + it, 1u);
             else
                 atomicAdd(hist + it, 1ull);
+            if (!it) capture_failure(cap, first_cw + cw);
         }
         wave_lds_fence();
     }
@@ -420,10 +421,10 @@ hipError_t t12_launch_decode(const uint16_t *llr_in, uint16_t *llr_out, uint32_t
 
 hipError_t t12_launch_mc(uint64_t seed, uint64_t first_cw, uint32_t n_cw, float mu, float sigma, int max_iter,
                          float beta, uint32_t *queue, unsigned long long *counters, unsigned long long *hist, int grid,
-                         hipStream_t st) {
+                         const Capture &cap, hipStream_t st) {
     const float beta3 = beta < 1.0f ? beta : 1.0f;
     hipLaunchKernelGGL(k_monte_carlo, dim3(grid), dim3(64 * WAVES_PER_WG), 0, st, seed, first_cw, n_cw, mu, sigma,
-                       max_iter, beta, beta3, queue, counters, hist);
+                       max_iter, beta, beta3, queue, counters, hist, cap);
     return hipGetLastError();
 }
 

@@ -5,6 +5,22 @@
 
 namespace ldpc {
 
+// Trapping-set capture: Monte Carlo kernels append the id of every codeword
+// that fails to ids[] (the first `max` of them, in completion order); the
+// channel and the decode of a captured id are reproduced from (seed, id).
+struct Capture {
+    uint32_t *count;
+    unsigned long long *ids;
+    uint32_t max;  // 0: off
+};
+
+__device__ __forceinline__ void capture_failure(const Capture &c, unsigned long long id) {
+    if (c.max) {
+        const uint32_t s = atomicAdd(c.count, 1u);
+        if (s < c.max) c.ids[s] = id;
+    }
+}
+
 // Profile T, AR4JA r1/2 K=1024 (decode_t.hip)
 int t12_waves_per_wg();
 hipError_t t12_occupancy(int *wg_per_cu);
@@ -13,7 +29,7 @@ hipError_t t12_launch_decode(const uint16_t *llr_in, uint16_t *llr_out, uint32_t
                              int max_iter, float beta, uint32_t *queue, int grid, hipStream_t st);
 hipError_t t12_launch_mc(uint64_t seed, uint64_t first_cw, uint32_t n_cw, float mu, float sigma, int max_iter,
                          float beta, uint32_t *queue, unsigned long long *counters, unsigned long long *hist, int grid,
-                         hipStream_t st);
+                         const Capture &cap, hipStream_t st);
 hipError_t t12_launch_gen(uint64_t seed, uint64_t first_cw, uint32_t n_cw, float mu, float sigma, uint16_t *out,
                           hipStream_t st);
 int t12_schedule(int32_t *out, int cap);
@@ -27,6 +43,7 @@ struct RLaunch {
     bool nms;
     int grid;
     uint32_t *queue;
+    Capture cap;
 };
 int r_waves_per_wg();
 size_t r_lds_bytes(int n, int n_edges);
@@ -65,6 +82,7 @@ struct MultiParams {
     const float *d_alpha, *d_beta;
     int d_groups, d_corr;
     float d_rmax;
+    Capture cap;  // Monte Carlo mode only
 };
 struct MultiVariant {
     int kid, rid, c, order;  // code (K = 1024 << 2 kid, rate 1/2, 2/3, 4/5), checks per thread, layer order

@@ -41,6 +41,10 @@ struct ldpc_ctx {
     // Profile T on the generic nine-code kernels (null: the specialised
     // AR4JA r1/2 K=1024 kernel), or the Profile D kernel
     const ldpc::MultiVariant *mv = nullptr;
+    // trapping-set capture (ldpc_set_capture)
+    uint32_t *d_cap_count = nullptr;
+    unsigned long long *d_cap_ids = nullptr;
+    uint32_t cap_max = 0;
     // Profile D parameters on the device
     float *d_alpha = nullptr, *d_beta = nullptr;
     int d_groups = 0, d_corr = 0;
@@ -415,6 +419,8 @@ void ldpc_destroy(ldpc_ctx *c) {
         if (b->p) (void)hipFree(b->p);
     if (c->d_table) (void)hipFree(c->d_table);
     if (c->d_alpha) (void)hipFree(c->d_alpha);
+    if (c->d_cap_count) (void)hipFree(c->d_cap_count);
+    if (c->d_cap_ids) (void)hipFree(c->d_cap_ids);
     if (c->d_beta) (void)hipFree(c->d_beta);
     if (c->d_queue) (void)hipFree(c->d_queue);
     if (c->d_counters) (void)hipFree(c->d_counters);
@@ -471,6 +477,43 @@ int ldpc_channel_params(const ldpc_ctx *c, double ebn0_db, float *mu, float *sig
     return LDPC_OK;
 }
 
+int ldpc_set_capture(ldpc_ctx *c, uint32_t max_frames) {
+    if (!c) return LDPC_ERR_ARG;
+    if (max_frames > (1u << 24)) return fail(c, LDPC_ERR_ARG, "max_frames must be <= 2^24");
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (c->d_cap_ids) HIP_TRY(c, hipFree(c->d_cap_ids));
+    c->d_cap_ids = nullptr;
+    c->cap_max = 0;
+    if (!max_frames) return LDPC_OK;
+    if (!c->d_cap_count) HIP_TRY(c, hipMalloc(&c->d_cap_count, sizeof(uint32_t)));
+    HIP_TRY(c, hipMalloc(&c->d_cap_ids, (size_t)max_frames * sizeof(unsigned long long)));
+    HIP_TRY(c, ldpc::launch_fill_u32(c->d_cap_count, 1, 0u, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    c->cap_max = max_frames;
+    return LDPC_OK;
+}
+
+int ldpc_get_captures(ldpc_ctx *c, uint64_t *cw_ids, uint32_t cap, uint64_t *n_failed) {
+    if (!c || (!cw_ids && cap)) return LDPC_ERR_ARG;
+    if (!c->cap_max) {
+        if (n_failed) *n_failed = 0;
+        return 0;
+    }
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    uint32_t count = 0;
+    HIP_TRY(c, hipMemcpy(&count, c->d_cap_count, sizeof(count), hipMemcpyDeviceToHost));
+    const uint32_t kept = std::min(count, c->cap_max);
+    std::vector<uint64_t> ids(kept);
+    if (kept) HIP_TRY(c, hipMemcpy(ids.data(), c->d_cap_ids, kept * sizeof(uint64_t), hipMemcpyDeviceToHost));
+    std::sort(ids.begin(), ids.end());
+    const uint32_t n = std::min(kept, cap);
+    std::copy(ids.begin(), ids.begin() + n, cw_ids);
+    if (n_failed) *n_failed = count;
+    return (int)n;
+}
+
 int ldpc_set_dmin_params(ldpc_ctx *c, const float *alpha, const float *beta, uint32_t groups, uint32_t n_params,
                          int correction, float r_max) {
     if (!c) return LDPC_ERR_ARG;
@@ -516,6 +559,7 @@ static ldpc::MultiParams multi_params(const ldpc_ctx *c) {
     p.d_groups = c->d_groups;
     p.d_corr = c->d_corr;
     p.d_rmax = c->d_rmax;
+    p.cap = ldpc::Capture{c->d_cap_count, c->d_cap_ids, c->cap_max};
     p.queue = c->d_queue;
     p.counters = c->d_counters;
     p.hist = c->d_hist;
@@ -539,6 +583,7 @@ int ldpc_run_point(ldpc_ctx *c, const ldpc_point_cfg *pt, ldpc_point_result *res
     const bool prof_t = c->cfg.profile == LDPC_PROFILE_T;
     ldpc::RLaunch rl = c->rl;
     rl.l_max = l_max_at(c, mu);
+    rl.cap = ldpc::Capture{c->d_cap_count, c->d_cap_ids, c->cap_max};
     const uint32_t nh = c->cfg.max_iter + 1;
     std::vector<unsigned long long> h(nh);
     unsigned long long cnt[4];
@@ -561,7 +606,7 @@ int ldpc_run_point(ldpc_ctx *c, const ldpc_point_cfg *pt, ldpc_point_result *res
             HIP_TRY(c, c->mv->launch(p, c->grid_mc, c->stream));
         } else if (prof_t)
             HIP_TRY(c, ldpc::t12_launch_mc(pt->seed, next, (uint32_t)n, mu, sigma, (int)c->cfg.max_iter, c->cfg.beta,
-                                           c->d_queue, c->d_counters, c->d_hist, c->grid_mc, c->stream));
+                                           c->d_queue, c->d_counters, c->d_hist, c->grid_mc, rl.cap, c->stream));
         else
             HIP_TRY(c, ldpc::r_launch_mc(rl, pt->seed, next, (uint32_t)n, mu, sigma, c->d_counters, c->d_hist,
                                          c->stream));
@@ -605,6 +650,7 @@ int ldpc_decode_llrs_device_async(ldpc_ctx *c, const void *d_llr, uint32_t n_cw,
         HIP_TRY(c, ldpc::launch_fill_u64(c->d_counters, 4, 0ull, c->stream));
         HIP_TRY(c, ldpc::launch_fill_u64(c->d_hist, c->cfg.max_iter + 1, 0ull, c->stream));
         ldpc::MultiParams p = multi_params(c);
+        p.cap = ldpc::Capture{nullptr, nullptr, 0};
         p.llr_in = (const uint16_t *)d_llr;
         p.llr_out = (uint16_t *)d_out_llr;
         p.iters = (uint32_t *)d_out_iters;

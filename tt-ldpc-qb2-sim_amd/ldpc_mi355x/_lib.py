@@ -90,6 +90,8 @@ SIGNATURES = {
                                                 C.c_void_p]),
     "ldpc_gen_channel": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, C.c_uint32, C.c_double, C.c_void_p]),
     "ldpc_channel_params": (C.c_int, [C.c_void_p, C.c_double, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+    "ldpc_set_capture": (C.c_int, [C.c_void_p, C.c_uint32]),
+    "ldpc_get_captures": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.POINTER(C.c_uint64)]),
     "ldpc_set_dmin_params": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_int,
                                        C.c_float]),
     "ldpc_sync": (C.c_int, [C.c_void_p]),

@@ -203,6 +203,19 @@ class Engine:
         self._check(self._lib.ldpc_channel_params(self._h, ebn0_db, C.byref(mu), C.byref(sg)))
         return mu.value, sg.value
 
+    def set_capture(self, max_frames: int) -> None:
+        """Record the ids of up to max_frames failing codewords in later
+        run_point calls (0: off)."""
+        self._check(self._lib.ldpc_set_capture(self._h, max_frames))
+
+    def captures(self, cap: int = 1 << 16):
+        """(ascending ids of captured failures, failures seen)."""
+        ids = np.zeros(cap, dtype=np.uint64)
+        seen = C.c_uint64()
+        n = self._lib.ldpc_get_captures(self._h, ids.ctypes.data, cap, C.byref(seen))
+        self._check(min(n, 0))
+        return ids[:n], int(seen.value)
+
     def set_dmin_params(self, alpha, beta, correction: bool = False, r_max: float = 0.0) -> None:
         """Profile D scale / offset rows [groups, n_params] (float32)."""
         a = np.ascontiguousarray(alpha, dtype=np.float32)

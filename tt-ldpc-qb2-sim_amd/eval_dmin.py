@@ -3,11 +3,16 @@
 
 Runs libldpc Profile D (the trained parameters, syndrome early exit) at each
 Eb/N0 for the given iteration caps, and Profile T (the golden Approx-Min*)
-at 200 iterations on the same codeword ids and seed -- the same channel
-realisations.  Prints one JSON line per (decoder, Eb/N0) with FER, its 95%
-interval, mean iterations and Gsps, then one summary line with the Eb/N0
-gap at equal FER (log-linear interpolation of each FER curve) and the
-throughput ratio.
+at 200 iterations (or ``--ref-iters``) on the same codeword ids and seed --
+the same channel realisations.  Codes without a Profile D kernel (K=16384
+r1/2, whose fp32 codeword does not fit in LDS) are decoded by the
+differentiable decoder's own forward kernels instead (``--forward``: T
+iterations, no early exit, frame error = an information bit <= 0), on the
+channel libldpc generates for the same ids.
+
+Prints one JSON line per (decoder, Eb/N0) with FER, its 95% interval, mean
+iterations and Gsps, then one summary line with the Eb/N0 gap at equal FER
+(log-linear interpolation of each FER curve).
 """
 import argparse
 import json
@@ -22,6 +27,36 @@ if HERE not in sys.path:
     sys.path.insert(0, HERE)
 
 from ldpc_mi355x import decoder  # noqa: E402
+
+
+def run_forward(z, ebn0, blocks, seed, target_fe, chunk=1024):
+    """The trained decoder through the extension's forward (no early exit)."""
+    import time
+
+    import torch  # loaded before libldpc's first use in this process (same HIP runtime)
+    from ldpc_mi355x import dmin
+    k, rate, T = int(z["k_info"]), str(z["rate"]), int(z["T"])
+    model = dmin.DMinSum(k, rate, T=T, correction=bool(z["correction"]), r_max=float(z["r_max"]),
+                         per_iteration=z["alpha"].shape[0] > 1)
+    with torch.no_grad():
+        model.alpha.copy_(torch.from_numpy(z["alpha"]))
+        model.beta.copy_(torch.from_numpy(z["beta"]))
+    gen = decoder.Engine(k, rate, max_iter=1)
+    fe = be = done = 0
+    t0 = time.time()
+    while done < blocks and (not target_fe or fe < target_fe):
+        n = min(chunk, blocks - done)
+        llr16 = torch.from_numpy(gen.gen_channel(seed, done, n, ebn0).astype(np.int32))
+        llr = (llr16 << 16).view(torch.float32).cuda()
+        with torch.no_grad():
+            L = model(llr)
+        bad = (L[:, :k] <= 0)
+        fe += int(bad.any(dim=1).sum())
+        be += int(bad.sum())
+        done += n
+    torch.cuda.synchronize()
+    gen.close()
+    return done, fe, be, time.time() - t0, model.n_tx
 
 
 def poisson_ci95(k):
@@ -46,15 +81,32 @@ def main(argv=None):
     ap.add_argument("--target-fe", type=int, default=0, help="stop a point early after this many frame errors")
     ap.add_argument("--iters", type=int, nargs="+", default=[20, 100], help="Profile D iteration caps")
     ap.add_argument("--seed", type=int, default=0x5EED2026)
+    ap.add_argument("--ref-iters", type=int, nargs="+", default=[200], help="Approx-Min* iteration caps")
+    ap.add_argument("--forward", action="store_true",
+                    help="decode the trained decoder with its training forward (T iterations, no early exit)")
     args = ap.parse_args(argv)
 
     z = np.load(args.params, allow_pickle=False)
     k, rate = int(z["k_info"]), str(z["rate"])
     curves = {}
-    runs = [("approx_min_star", 200, None)] + [(f"trained_T{t}", t, t) for t in args.iters]
+    if args.forward:
+        import torch  # noqa: F401  (before libldpc: one HIP runtime for both)
+        pts = []
+        for eb in args.ebn0:
+            done, fe, be, sec, n_tx = run_forward(z, eb, args.blocks, args.seed, args.target_fe)
+            lo, hi = poisson_ci95(fe)
+            pts.append((eb, fe / done))
+            print(json.dumps({"decoder": f"trained_forward_T{int(z['T'])}", "max_iter": int(z["T"]), "ebn0_db": eb,
+                              "blocks": done, "frame_errors": fe, "fer": fe / done,
+                              "fer_ci95": [lo / done, hi / done], "ber": be / (done * k),
+                              "gsps": done * n_tx / sec / 1e9}), flush=True)
+        curves[f"trained_forward_T{int(z['T'])}"] = pts
+    runs = [(f"approx_min_star_{r}" if r != 200 else "approx_min_star", r, None) for r in args.ref_iters]
+    if not args.forward:
+        runs += [(f"trained_T{t}", t, t) for t in args.iters]
     for name, max_iter, _ in runs:
-        if name == "approx_min_star":
-            eng = decoder.Engine(k, rate, max_iter=200)
+        if name.startswith("approx_min_star"):
+            eng = decoder.Engine(k, rate, max_iter=max_iter)
         else:
             eng = decoder.Engine(k, rate, max_iter=max_iter, profile="D")
             eng.set_dmin_params(z["alpha"], z["beta"], bool(z["correction"]), float(z["r_max"]))
@@ -69,10 +121,11 @@ def main(argv=None):
                               "mean_iterations": r.mean_iterations, "kernel_gsps": r.kernel_gsps}), flush=True)
         curves[name] = pts
         eng.close()
-    ref = curves["approx_min_star"]
-    summary = {}
+    ref_name = [n for n in curves if n.startswith("approx_min_star")][-1]
+    ref = curves[ref_name]
+    summary = {"reference": ref_name}
     for name, pts in curves.items():
-        if name == "approx_min_star":
+        if name.startswith("approx_min_star"):
             continue
         gaps = []
         for _, f in ref:

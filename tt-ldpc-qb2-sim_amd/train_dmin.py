@@ -6,8 +6,9 @@ unrolled iterations, one FP32 scale and offset per (layer, circulant),
 initialised at 1 / 0, batches of 65536 codewords at Eb/N0 uniform in
 [1.0, 2.5] dB, binary cross-entropy of the soft output against the
 transmitted (all-zero) bits, Adam at 1e-3, checkpoints every 4 layer steps.
-The large configuration is ``--k 16384 --T 50 --ckpt-every 600 --batch 4096
---bf16-llr``.
+The large configuration is ``--k 16384 --T 50 --ckpt-every 60 --batch 4096
+--micro-batch 1024 --bf16-llr`` (recompute from a checkpoint every 5
+iterations).
 
 Writes the parameters (numpy .npz: alpha, beta and the decoder settings)
 and prints one JSON line per validation.  The validation set is fixed

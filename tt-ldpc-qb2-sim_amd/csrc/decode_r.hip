@@ -52,71 +52,89 @@ __device__ __forceinline__ float corr_r(float diff) {
 
 // One fold step: max(min(m, a) - g(|m - a|), 0).  m, a >= +0 and the
 // difference is never -0, so fmaxf agrees with the reference's select.
+// For non-negative finite floats the bit patterns order like the values, so
+// min(m, a) is an unsigned integer min (no canonicalisation of the inputs).
 __device__ __forceinline__ float fold_r(float m, float a) {
     const float diff = fabsf(__fsub_rn(m, a));
-    const float cur = fminf(m, a);
+    const float cur = u2f(min(f2u(m), f2u(a)));
     return fmaxf(__fsub_rn(cur, corr_r(diff)), 0.0f);
 }
 
+// One lane's entry of the pass table as loaded: word 0 = degree |
+// edge_start << 4, then the variable indices as 16-bit pairs (4 words per
+// entry for MAXD 6, 8 for MAXD 12).  Slots past
+// the degree (and idle lanes, degree 0) hold variable 0: a valid address
+// whose value is read and then masked out, so every gather of a row can be
+// issued before the first one is used.
 template <int MAXD>
-struct Row {
-    uint32_t meta;  // degree | edge_start << 4 (degree 0: idle lane)
-    uint16_t v[MAXD];
+struct RowRaw {
+    uint4 w[MAXD / 6];
 };
 
 template <int MAXD>
-__device__ __forceinline__ Row<MAXD> load_row(const uint4 *table, int pass, int lane) {
-    Row<MAXD> r;
-    const uint4 a = table[(size_t)(pass * 64 + lane) * (MAXD / 6)];
-    r.meta = a.x;
-    r.v[0] = (uint16_t)a.y;
-    r.v[1] = (uint16_t)(a.y >> 16);
-    r.v[2] = (uint16_t)a.z;
-    r.v[3] = (uint16_t)(a.z >> 16);
-    r.v[4] = (uint16_t)a.w;
-    r.v[5] = (uint16_t)(a.w >> 16);
-    if constexpr (MAXD == 12) {
-        const uint4 b = table[(size_t)(pass * 64 + lane) * 2 + 1];
-        r.v[6] = (uint16_t)b.x;
-        r.v[7] = (uint16_t)(b.x >> 16);
-        r.v[8] = (uint16_t)b.y;
-        r.v[9] = (uint16_t)(b.y >> 16);
-        r.v[10] = (uint16_t)b.z;
-        r.v[11] = (uint16_t)(b.z >> 16);
-    }
+__device__ __forceinline__ RowRaw<MAXD> load_row(const uint4 *table, int pass, int lane) {
+    RowRaw<MAXD> r;
+#pragma unroll
+    for (int k = 0; k < MAXD / 6; ++k) r.w[k] = table[(size_t)(pass * 64 + lane) * (MAXD / 6) + k];
     return r;
+}
+
+// Variable d sits in 32-bit word 1 + d / 2 of the entry (words 1..3 for
+// d < 6, words 4..6 for 6 <= d < 12), low half for even d.
+template <int MAXD>
+__device__ __forceinline__ int var_of(const RowRaw<MAXD> &r, int d) {
+    const int k = 1 + d / 2;
+    const uint4 &w = r.w[k / 4];
+    const uint32_t word = (k % 4) == 0 ? w.x : (k % 4) == 1 ? w.y : (k % 4) == 2 ? w.z : w.w;
+    return (int)((d & 1) ? (word >> 16) : (word & 0xFFFFu));
 }
 
 struct RCfg {
     float r_max, l_post_max;
 };
 
-// One row on one lane (inactive lanes have deg == 0 and do nothing).
-template <int MAXD, bool NMS>
-__device__ __forceinline__ void row_update(uint16_t *Ls, uint16_t *Rs, const Row<MAXD> &row, const RCfg &cfg) {
-    const int deg = (int)(row.meta & 15u);
-    const int es = (int)(row.meta >> 4);
-    if (deg == 0) return;
+// One row on one lane.  Straight-line code: all 2 * MAXD gathers first,
+// then the arithmetic with slots d >= deg masked by selects, then the
+// scatters of the deg real slots.  Idle lanes (deg == 0) compute on
+// variable 0 / edge 0 and write nothing.  The host lays the messages out so
+// that es + MAXD - 1 is inside the message array for every row (a row of
+// the largest degree is stored last, plus MAXD - maxd spare slots), so the
+// message reads of slots past the degree need no address select.
+// CLAMP: the optional |R| <= r_max and |L'| <= l_post_max clamps.
+template <int MAXD, bool NMS, bool CLAMP>
+__device__ __forceinline__ void row_update(uint16_t *Ls, uint16_t *Rs, const RowRaw<MAXD> &row, const RCfg &cfg) {
+    const uint32_t meta = row.w[0].x;
+    const int deg = (int)(meta & 15u);
+    const int es = (int)(meta >> 4);
+    int v[MAXD];
+    uint32_t lh[MAXD], rh[MAXD];
+#pragma unroll
+    for (int d = 0; d < MAXD; ++d) v[d] = var_of<MAXD>(row, d);
+#pragma unroll
+    for (int d = 0; d < MAXD; ++d) {
+        lh[d] = Ls[v[d]];
+        rh[d] = Rs[es + d];
+    }
+    __builtin_amdgcn_sched_barrier(0);  // keep the 2 * MAXD reads in one batch
     float q[MAXD];
-    uint32_t sign = 0;
+    uint32_t sign = 0;  // bit 31: parity of the signs of q over the row
+#pragma unroll
+    for (int d = 0; d < MAXD; ++d) {
+        q[d] = __fsub_rn(u2f(lh[d] << 16), u2f(rh[d] << 16));
+        sign ^= d < deg ? f2u(q[d]) : 0u;
+    }
     int idx = 0;
     float r_min, r_all;
     if constexpr (NMS) {
         float min1 = 999.0f, min2 = 999.0f;
 #pragma unroll
         for (int d = 0; d < MAXD; ++d) {
-            if (d < deg) {
-                q[d] = __fsub_rn(bf(Ls[row.v[d]]), bf(Rs[es + d]));
-                sign ^= f2u(q[d]) >> 31;
-                const float a = absbits(q[d]);
-                if (a < min1) {
-                    min2 = min1;
-                    min1 = a;
-                    idx = d;
-                } else if (a < min2) {
-                    min2 = a;
-                }
-            }
+            const float a = absbits(q[d]);
+            const bool c1 = d < deg && a < min1;
+            const bool c2 = d < deg && !c1 && a < min2;
+            min2 = c1 ? min1 : (c2 ? a : min2);
+            min1 = c1 ? a : min1;
+            idx = c1 ? d : idx;
         }
         r_min = __fmul_rn(0.75f, min2);
         r_all = __fmul_rn(0.75f, min1);
@@ -124,57 +142,70 @@ __device__ __forceinline__ void row_update(uint16_t *Ls, uint16_t *Rs, const Row
         float min_mag = 9999.0f;
 #pragma unroll
         for (int d = 0; d < MAXD; ++d) {
-            if (d < deg) {
-                q[d] = __fsub_rn(bf(Ls[row.v[d]]), bf(Rs[es + d]));
-                sign ^= f2u(q[d]) >> 31;
-                const float a = absbits(q[d]);
-                if (a < min_mag) {
-                    min_mag = a;
-                    idx = d;
-                }
-            }
+            const float a = absbits(q[d]);
+            const bool c = d < deg && a < min_mag;
+            min_mag = c ? a : min_mag;
+            idx = c ? d : idx;
         }
         // fold the non-minimum edges in index order (ldpc_decoder_core.h:180-190)
         const int first = idx == 0 ? 1 : 0;
-        float m = deg > 1 ? absbits(q[first]) : min_mag;
+        float m = deg > 1 ? absbits(first ? q[1] : q[0]) : min_mag;
 #pragma unroll
-        for (int d = 1; d < MAXD; ++d)
-            if (d > first && d < deg && d != idx) m = fold_r(m, absbits(q[d]));
+        for (int d = 1; d < MAXD; ++d) {
+            const float f = fold_r(m, absbits(q[d]));
+            m = (d > first && d < deg && d != idx) ? f : m;
+        }
         r_min = m;
         r_all = fold_r(m, min_mag);
     }
+    if (CLAMP && cfg.r_max > 0.0f) {
+        r_min = r_min > cfg.r_max ? cfg.r_max : r_min;
+        r_all = r_all > cfg.r_max ? cfg.r_max : r_all;
+    }
+    // R' = bf16(rn) = sign | bf16(|rn|): two magnitudes per row
+    const uint32_t rmin_b = f2u(r_min), rall_b = f2u(r_all);
+    const uint32_t rmin_h = rne_bits(r_min), rall_h = rne_bits(r_all);
 #pragma unroll
     for (int d = 0; d < MAXD; ++d) {
+        const bool is_min = d == idx;
+        const uint32_t s = (sign ^ f2u(q[d])) & 0x80000000u;
+        const float rn = u2f((is_min ? rmin_b : rall_b) | s);
+        float ln = __fadd_rn(q[d], rn);
+        if (CLAMP && cfg.l_post_max > 0.0f) ln = __builtin_amdgcn_fmed3f(ln, -cfg.l_post_max, cfg.l_post_max);
         if (d < deg) {
-            float mag = d == idx ? r_min : r_all;
-            if (cfg.r_max > 0.0f && mag > cfg.r_max) mag = cfg.r_max;
-            const uint32_t s = sign ^ (f2u(q[d]) >> 31);
-            const float rn = u2f(f2u(mag) | (s << 31));
-            float ln = __fadd_rn(q[d], rn);
-            if (cfg.l_post_max > 0.0f) {
-                if (ln > cfg.l_post_max)
-                    ln = cfg.l_post_max;
-                else if (ln < -cfg.l_post_max)
-                    ln = -cfg.l_post_max;
-            }
-            Ls[row.v[d]] = (uint16_t)(rne_bits(ln) >> 16);
-            Rs[es + d] = (uint16_t)(rne_bits(rn) >> 16);
+            Ls[v[d]] = (uint16_t)(rne_bits(ln) >> 16);
+            Rs[es + d] = (uint16_t)(((is_min ? rmin_h : rall_h) | s) >> 16);
         }
     }
 }
 
+// Parity of a row on the raw sign bits (slots past the degree masked).
+template <int MAXD>
+__device__ __forceinline__ uint32_t row_parity(const uint16_t *Ls, const RowRaw<MAXD> &row) {
+    const int deg = (int)(row.w[0].x & 15u);
+    uint32_t h[MAXD];
+#pragma unroll
+    for (int d = 0; d < MAXD; ++d) h[d] = Ls[var_of<MAXD>(row, d)];
+    __builtin_amdgcn_sched_barrier(0);
+    uint32_t par = 0;
+#pragma unroll
+    for (int d = 0; d < MAXD; ++d) par ^= d < deg ? h[d] : 0u;
+    return (par >> 15) & 1u;
+}
+
 // Syndrome with the hard decision on the raw sign bit; stops at the first
-// pass that holds an unsatisfied row.
+// pass that holds an unsatisfied row.  The table entry of the next pass is
+// loaded while the current one is checked (two named buffers, no copies).
 template <int MAXD>
 __device__ __forceinline__ bool syndrome_ok(const uint16_t *Ls, const uint4 *table, int n_pass, int lane) {
-    for (int p = 0; p < n_pass; ++p) {
-        const Row<MAXD> row = load_row<MAXD>(table, p, lane);
-        const int deg = (int)(row.meta & 15u);
-        uint32_t par = 0;
-#pragma unroll
-        for (int d = 0; d < MAXD; ++d)
-            if (d < deg) par ^= (uint32_t)Ls[row.v[d]] >> 15;
-        if (__ballot(par & 1u)) return false;
+    const int last = n_pass - 1;
+    RowRaw<MAXD> x = load_row<MAXD>(table, 0, lane), y;
+    for (int p = 0; p <= last; p += 2) {
+        y = load_row<MAXD>(table, min(p + 1, last), lane);
+        if (__ballot(row_parity<MAXD>(Ls, x))) return false;
+        if (p + 1 > last) break;
+        x = load_row<MAXD>(table, min(p + 2, last), lane);
+        if (__ballot(row_parity<MAXD>(Ls, y))) return false;
     }
     return true;
 }
@@ -186,17 +217,22 @@ struct RArgs {
     float r_max, l_post_max, l_max;
 };
 
-template <int MAXD, bool NMS>
+template <int MAXD, bool NMS, bool CLAMP>
 __device__ int decode_wave_r(uint16_t *Ls, uint16_t *Rs, int lane, const RArgs &a) {
     for (int e = lane; e < a.n_edges; e += 64) Rs[e] = 0;
     wave_fence();
     const RCfg cfg{a.r_max, a.l_post_max};
+    const int last = a.n_pass - 1;
     for (int it = 0; it < a.max_iter; ++it) {
-        Row<MAXD> row = load_row<MAXD>(a.table, 0, lane);
-        for (int p = 0; p < a.n_pass; ++p) {
-            const Row<MAXD> cur = row;
-            if (p + 1 < a.n_pass) row = load_row<MAXD>(a.table, p + 1, lane);  // prefetch
-            row_update<MAXD, NMS>(Ls, Rs, cur, cfg);
+        // pass p + 1's entry is in flight while pass p runs
+        RowRaw<MAXD> x = load_row<MAXD>(a.table, 0, lane), y;
+        for (int p = 0; p <= last; p += 2) {
+            y = load_row<MAXD>(a.table, min(p + 1, last), lane);
+            row_update<MAXD, NMS, CLAMP>(Ls, Rs, x, cfg);
+            wave_fence();
+            if (p + 1 > last) break;
+            x = load_row<MAXD>(a.table, min(p + 2, last), lane);
+            row_update<MAXD, NMS, CLAMP>(Ls, Rs, y, cfg);
             wave_fence();
         }
         if (it >= 2 && syndrome_ok<MAXD>(Ls, a.table, a.n_pass, lane)) return it + 1;
@@ -244,7 +280,7 @@ __device__ __forceinline__ uint32_t next_cw(uint32_t *queue, int lane) {
     return __builtin_amdgcn_readfirstlane(v);
 }
 
-template <int MAXD, bool NMS>
+template <int MAXD, bool NMS, bool CLAMP>
 __global__ void __launch_bounds__(64 * WAVES)
     k_decode_llrs_r(RArgs a, const uint16_t *__restrict__ llr_in, uint16_t *__restrict__ llr_out,
                     uint32_t *__restrict__ iters, uint8_t *__restrict__ ok, uint32_t n_cw, uint32_t *queue) {
@@ -258,7 +294,7 @@ __global__ void __launch_bounds__(64 * WAVES)
         const uint16_t *src = llr_in + (size_t)cw * a.n;
         for (int i = lane; i < a.n; i += 64) Ls[i] = src[i];
         wave_fence();
-        const int it = decode_wave_r<MAXD, NMS>(Ls, Rs, lane, a);
+        const int it = decode_wave_r<MAXD, NMS, CLAMP>(Ls, Rs, lane, a);
         wave_fence();
         uint16_t *dst = llr_out + (size_t)cw * a.n;
         for (int i = lane; i < a.n; i += 64) dst[i] = Ls[i];
@@ -270,7 +306,7 @@ __global__ void __launch_bounds__(64 * WAVES)
     }
 }
 
-template <int MAXD, bool NMS>
+template <int MAXD, bool NMS, bool CLAMP>
 __global__ void __launch_bounds__(64 * WAVES)
     k_monte_carlo_r(RArgs a, uint64_t seed, uint64_t first_cw, uint32_t n_cw, float mu, float sigma, uint32_t *queue,
                     unsigned long long *counters, unsigned long long *hist, Capture cap) {
@@ -284,7 +320,7 @@ __global__ void __launch_bounds__(64 * WAVES)
         if (cw >= n_cw) break;
         wave_gen_channel_r(Ls, lane, seed, first_cw + cw, mu, sigma, a.l_max, a.n_tx, a.n);
         wave_fence();
-        const int it = decode_wave_r<MAXD, NMS>(Ls, Rs, lane, a);
+        const int it = decode_wave_r<MAXD, NMS, CLAMP>(Ls, Rs, lane, a);
         wave_fence();
         uint32_t b = 0;
         for (int i0 = 0; i0 < a.n_tx; i0 += 64) {
@@ -309,22 +345,22 @@ __global__ void __launch_bounds__(64 * WAVES)
     }
 }
 
-template <int MAXD, bool NMS>
+template <int MAXD, bool NMS, bool CLAMP>
 hipError_t launch_decode_t(const RLaunch &L, const uint16_t *llr_in, uint16_t *llr_out, uint32_t *iters, uint8_t *ok,
                            uint32_t n_cw, hipStream_t st) {
     RArgs a{(const uint4 *)L.table, L.n_pass, L.n, L.n_tx, L.n_edges, L.max_iter, L.r_max, L.l_post_max, L.l_max};
     const size_t lds = (size_t)WAVES * 2 * (((L.n + L.n_edges) + 7) & ~7);
-    hipLaunchKernelGGL((k_decode_llrs_r<MAXD, NMS>), dim3(L.grid), dim3(64 * WAVES), lds, st, a, llr_in, llr_out, iters,
+    hipLaunchKernelGGL((k_decode_llrs_r<MAXD, NMS, CLAMP>), dim3(L.grid), dim3(64 * WAVES), lds, st, a, llr_in, llr_out, iters,
                        ok, n_cw, L.queue);
     return hipGetLastError();
 }
 
-template <int MAXD, bool NMS>
+template <int MAXD, bool NMS, bool CLAMP>
 hipError_t launch_mc_t(const RLaunch &L, uint64_t seed, uint64_t first_cw, uint32_t n_cw, float mu, float sigma,
                        unsigned long long *counters, unsigned long long *hist, hipStream_t st) {
     RArgs a{(const uint4 *)L.table, L.n_pass, L.n, L.n_tx, L.n_edges, L.max_iter, L.r_max, L.l_post_max, L.l_max};
     const size_t lds = (size_t)WAVES * 2 * (((L.n + L.n_edges) + 7) & ~7);
-    hipLaunchKernelGGL((k_monte_carlo_r<MAXD, NMS>), dim3(L.grid), dim3(64 * WAVES), lds, st, a, seed, first_cw, n_cw,
+    hipLaunchKernelGGL((k_monte_carlo_r<MAXD, NMS, CLAMP>), dim3(L.grid), dim3(64 * WAVES), lds, st, a, seed, first_cw, n_cw,
                        mu, sigma, L.queue, counters, hist, L.cap);
     return hipGetLastError();
 }
@@ -369,6 +405,23 @@ __global__ void __launch_bounds__(256)
     }
 }
 
+template <int MAXD, bool NMS, bool CLAMP>
+hipError_t occ_t(size_t lds, int *wg_per_cu) {
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(wg_per_cu, k_monte_carlo_r<MAXD, NMS, CLAMP>, 64 * WAVES, lds);
+}
+
+// Returns CALL(MAXD, NMS, CLAMP) for the launch's configuration.
+#define R_DISPATCH(L, CALL)                                             \
+    do {                                                                \
+        const bool c_ = (L).r_max > 0.0f || (L).l_post_max > 0.0f;      \
+        if ((L).maxd <= 6) {                                            \
+            if ((L).nms) return c_ ? CALL(6, true, true) : CALL(6, true, false);    \
+            return c_ ? CALL(6, false, true) : CALL(6, false, false);               \
+        }                                                               \
+        if ((L).nms) return c_ ? CALL(12, true, true) : CALL(12, true, false);      \
+        return c_ ? CALL(12, false, true) : CALL(12, false, false);                 \
+    } while (0)
+
 }  // namespace
 
 hipError_t r_launch_gen(uint64_t seed, uint64_t first_cw, uint32_t n_cw, float mu, float sigma, float l_max, int n_tx,
@@ -385,34 +438,24 @@ int r_waves_per_wg() { return WAVES; }
 
 size_t r_lds_bytes(int n, int n_edges) { return (size_t)WAVES * 2 * (((n + n_edges) + 7) & ~7); }
 
-hipError_t r_occupancy(int maxd, bool nms, size_t lds, int *wg_per_cu) {
-#define OCC(M, N) \
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(wg_per_cu, k_monte_carlo_r<M, N>, 64 * WAVES, lds)
-    if (maxd <= 6) {
-        if (nms) OCC(6, true);
-        OCC(6, false);
-    }
-    if (nms) OCC(12, true);
-    OCC(12, false);
+hipError_t r_occupancy(const RLaunch &L, size_t lds, int *wg_per_cu) {
+#define OCC(M, N, C) occ_t<M, N, C>(lds, wg_per_cu)
+    R_DISPATCH(L, OCC);
 #undef OCC
 }
 
 hipError_t r_launch_decode(const RLaunch &L, const uint16_t *llr_in, uint16_t *llr_out, uint32_t *iters, uint8_t *ok,
                            uint32_t n_cw, hipStream_t st) {
-    if (L.maxd <= 6)
-        return L.nms ? launch_decode_t<6, true>(L, llr_in, llr_out, iters, ok, n_cw, st)
-                     : launch_decode_t<6, false>(L, llr_in, llr_out, iters, ok, n_cw, st);
-    return L.nms ? launch_decode_t<12, true>(L, llr_in, llr_out, iters, ok, n_cw, st)
-                 : launch_decode_t<12, false>(L, llr_in, llr_out, iters, ok, n_cw, st);
+#define DEC(M, N, C) launch_decode_t<M, N, C>(L, llr_in, llr_out, iters, ok, n_cw, st)
+    R_DISPATCH(L, DEC);
+#undef DEC
 }
 
 hipError_t r_launch_mc(const RLaunch &L, uint64_t seed, uint64_t first_cw, uint32_t n_cw, float mu, float sigma,
                        unsigned long long *counters, unsigned long long *hist, hipStream_t st) {
-    if (L.maxd <= 6)
-        return L.nms ? launch_mc_t<6, true>(L, seed, first_cw, n_cw, mu, sigma, counters, hist, st)
-                     : launch_mc_t<6, false>(L, seed, first_cw, n_cw, mu, sigma, counters, hist, st);
-    return L.nms ? launch_mc_t<12, true>(L, seed, first_cw, n_cw, mu, sigma, counters, hist, st)
-                 : launch_mc_t<12, false>(L, seed, first_cw, n_cw, mu, sigma, counters, hist, st);
+#define MC(M, N, C) launch_mc_t<M, N, C>(L, seed, first_cw, n_cw, mu, sigma, counters, hist, st)
+    R_DISPATCH(L, MC);
+#undef MC
 }
 
 }  // namespace ldpc

@@ -37,7 +37,8 @@ int t12_schedule(int32_t *out, int cap);
 // Profile R (decode_r.hip): level-scheduled rows, one row per lane.
 struct RLaunch {
     const void *table;  // n_pass * 64 entries of 16 B (maxd <= 6) or 32 B (maxd <= 12)
-    int n_pass, n, n_tx, n_edges, max_iter;
+    int n_pass, n, n_tx, max_iter;
+    int n_edges;  // message slots in LDS: the code's edges plus MAXD - maxd spare
     float r_max, l_post_max, l_max;
     int maxd;
     bool nms;
@@ -47,7 +48,7 @@ struct RLaunch {
 };
 int r_waves_per_wg();
 size_t r_lds_bytes(int n, int n_edges);
-hipError_t r_occupancy(int maxd, bool nms, size_t lds, int *wg_per_cu);
+hipError_t r_occupancy(const RLaunch &L, size_t lds, int *wg_per_cu);
 hipError_t r_launch_decode(const RLaunch &L, const uint16_t *llr_in, uint16_t *llr_out, uint32_t *iters, uint8_t *ok,
                            uint32_t n_cw, hipStream_t st);
 hipError_t r_launch_gen(uint64_t seed, uint64_t first_cw, uint32_t n_cw, float mu, float sigma, float l_max, int n_tx,

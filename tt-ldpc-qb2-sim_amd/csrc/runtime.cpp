@@ -152,14 +152,33 @@ int setup_profile_r(ldpc_ctx *c, const std::vector<int32_t> &rp, const std::vect
     if (rc) return fail(nullptr, rc, why);
     if (n > 65535) return fail(nullptr, LDPC_ERR_UNSUPPORTED, "Profile R: more than 65535 variables");
     const uint32_t n_edges = (uint32_t)rp[m];
-    if (n_edges >= (1u << 28) || ldpc::r_lds_bytes((int)n, (int)n_edges) > 65536)
-        return fail(nullptr, LDPC_ERR_UNSUPPORTED,
-                    "Profile R keeps a codeword's LLRs and messages in LDS: needs n + edges <= 16384");
-    std::vector<int32_t> level(m);
-    const int n_levels = levelize(n, m, rp.data(), col.data(), level.data());
     int maxd = 0;
     for (uint32_t i = 0; i < m; ++i) maxd = std::max(maxd, rp[i + 1] - rp[i]);
-    const int words = maxd <= 6 ? 4 : 8;  // uint32 per lane entry
+    const int kmaxd = maxd <= 6 ? 6 : 12;  // the kernel's slot count
+    // Message layout in LDS (private to the kernel, zeroed on entry): rows in
+    // file order, except that the last row of the largest degree is moved to
+    // the end, plus kmaxd - maxd spare slots.  Then es + kmaxd - 1 is a valid
+    // slot for every row, and the kernel reads all kmaxd slots unmasked.
+    const uint32_t r_slots = n_edges + (uint32_t)(kmaxd - maxd);
+    if (n_edges >= (1u << 28) || ldpc::r_lds_bytes((int)n, (int)r_slots) > 65536)
+        return fail(nullptr, LDPC_ERR_UNSUPPORTED,
+                    "Profile R keeps a codeword's LLRs and messages in LDS: needs n + edges <= 16384");
+    std::vector<uint32_t> r_off(m);
+    {
+        uint32_t last = 0;
+        for (uint32_t i = 0; i < m; ++i)
+            if (rp[i + 1] - rp[i] == maxd) last = i;
+        uint32_t off = 0;
+        for (uint32_t i = 0; i < m; ++i)
+            if (i != last) {
+                r_off[i] = off;
+                off += (uint32_t)(rp[i + 1] - rp[i]);
+            }
+        r_off[last] = off;
+    }
+    std::vector<int32_t> level(m);
+    const int n_levels = levelize(n, m, rp.data(), col.data(), level.data());
+    const int words = kmaxd == 6 ? 4 : 8;  // uint32 per lane entry
     std::vector<std::vector<uint32_t>> by_level(n_levels + 1);
     for (uint32_t i = 0; i < m; ++i) by_level[level[i]].push_back(i);
     std::vector<uint32_t> table;
@@ -173,7 +192,7 @@ int setup_profile_r(ldpc_ctx *c, const std::vector<int32_t> &rp, const std::vect
                 const uint32_t r = rows[off + k];
                 uint32_t *e = &table[base + k * words];
                 const uint32_t deg = (uint32_t)(rp[r + 1] - rp[r]);
-                e[0] = deg | ((uint32_t)rp[r] << 4);
+                e[0] = deg | (r_off[r] << 4);
                 // variables 0..5 in words 1..3, 6..11 in words 4..6
                 for (uint32_t d = 0; d < deg; ++d)
                     e[1 + d / 2] |= ((uint32_t)col[rp[r] + d] & 0xFFFFu) << (16 * (d & 1));
@@ -187,7 +206,7 @@ int setup_profile_r(ldpc_ctx *c, const std::vector<int32_t> &rp, const std::vect
     c->rl.n_pass = n_pass;
     c->rl.n = (int)n;
     c->rl.n_tx = (int)(n - np);
-    c->rl.n_edges = (int)n_edges;
+    c->rl.n_edges = (int)r_slots;
     c->rl.max_iter = (int)c->cfg.max_iter;
     c->rl.r_max = c->cfg.r_max;
     c->rl.l_post_max = c->cfg.l_post_max;
@@ -399,7 +418,7 @@ int ldpc_create(const ldpc_code_desc *code, const ldpc_decoder_cfg *cfg, int dev
             return rc;
         }
         int occ = 0;
-        if ((e = ldpc::r_occupancy(c->rl.maxd, c->rl.nms, ldpc::r_lds_bytes(c->rl.n, c->rl.n_edges), &occ)) !=
+        if ((e = ldpc::r_occupancy(c->rl, ldpc::r_lds_bytes(c->rl.n, c->rl.n_edges), &occ)) !=
             hipSuccess)
             return bail(e, "occupancy");
         c->grid_mc = c->grid_dec = cus * (occ > 0 ? occ : 1);

@@ -69,6 +69,8 @@ def main(argv=None):
     ap.add_argument("--val-every", type=int, default=25)
     ap.add_argument("--init-alpha", type=float, default=1.0)
     ap.add_argument("--init-beta", type=float, default=0.0)
+    ap.add_argument("--init-from", default=None,
+                    help=".npz written by this script: start from its alpha / beta (same code and parameter shape)")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--out", default="dmin_params.npz")
     args = ap.parse_args(argv)
@@ -78,6 +80,12 @@ def main(argv=None):
     with torch.no_grad():
         model.alpha.fill_(args.init_alpha)
         model.beta.fill_(args.init_beta)
+        if args.init_from:
+            z = np.load(args.init_from, allow_pickle=False)
+            if tuple(z["alpha"].shape) != tuple(model.alpha.shape):
+                raise SystemExit(f"--init-from: parameter shape {z['alpha'].shape} does not match {tuple(model.alpha.shape)}")
+            model.alpha.copy_(torch.from_numpy(z["alpha"]))
+            model.beta.copy_(torch.from_numpy(z["beta"]))
     gen = torch.Generator(device="cuda")
     gen.manual_seed(12345)
     sets = {eb: dmin.channel_llr(model.code, torch.full((args.val_size,), eb, device="cuda"), gen, args.bf16_llr)

@@ -29,13 +29,16 @@ if HERE not in sys.path:
 from ldpc_mi355x import decoder  # noqa: E402
 
 
-def run_forward(z, ebn0, blocks, seed, target_fe, chunk=1024):
-    """The trained decoder through the extension's forward (no early exit)."""
+def run_forward(z, ebn0, blocks, seed, target_fe, chunk=1024, T=0):
+    """The trained decoder through the extension's forward (no early exit),
+    T iterations (0: the T it was trained with)."""
     import time
 
     import torch  # loaded before libldpc's first use in this process (same HIP runtime)
     from ldpc_mi355x import dmin
-    k, rate, T = int(z["k_info"]), str(z["rate"]), int(z["T"])
+    k, rate, T = int(z["k_info"]), str(z["rate"]), (T or int(z["T"]))
+    if z["alpha"].shape[0] > 1 and T != int(z["T"]):
+        raise SystemExit("--forward-T needs shared (not per-iteration) parameters")
     model = dmin.DMinSum(k, rate, T=T, correction=bool(z["correction"]), r_max=float(z["r_max"]),
                          per_iteration=z["alpha"].shape[0] > 1)
     with torch.no_grad():
@@ -84,6 +87,7 @@ def main(argv=None):
     ap.add_argument("--ref-iters", type=int, nargs="+", default=[200], help="Approx-Min* iteration caps")
     ap.add_argument("--forward", action="store_true",
                     help="decode the trained decoder with its training forward (T iterations, no early exit)")
+    ap.add_argument("--forward-T", type=int, default=0, help="iterations of the --forward decode (0: the trained T)")
     args = ap.parse_args(argv)
 
     z = np.load(args.params, allow_pickle=False)
@@ -92,15 +96,16 @@ def main(argv=None):
     if args.forward:
         import torch  # noqa: F401  (before libldpc: one HIP runtime for both)
         pts = []
+        t_fwd = args.forward_T or int(z["T"])
         for eb in args.ebn0:
-            done, fe, be, sec, n_tx = run_forward(z, eb, args.blocks, args.seed, args.target_fe)
+            done, fe, be, sec, n_tx = run_forward(z, eb, args.blocks, args.seed, args.target_fe, T=t_fwd)
             lo, hi = poisson_ci95(fe)
             pts.append((eb, fe / done))
-            print(json.dumps({"decoder": f"trained_forward_T{int(z['T'])}", "max_iter": int(z["T"]), "ebn0_db": eb,
+            print(json.dumps({"decoder": f"trained_forward_T{t_fwd}", "max_iter": t_fwd, "ebn0_db": eb,
                               "blocks": done, "frame_errors": fe, "fer": fe / done,
                               "fer_ci95": [lo / done, hi / done], "ber": be / (done * k),
                               "gsps": done * n_tx / sec / 1e9}), flush=True)
-        curves[f"trained_forward_T{int(z['T'])}"] = pts
+        curves[f"trained_forward_T{t_fwd}"] = pts
     runs = [(f"approx_min_star_{r}" if r != 200 else "approx_min_star", r, None) for r in args.ref_iters]
     if not args.forward:
         runs += [(f"trained_T{t}", t, t) for t in args.iters]

@@ -98,21 +98,35 @@ struct Cfg {
     static constexpr int THREADS = 64 * WPW;
     static constexpr int CW_PER_WG = WAVE ? SLOTS * WPW : 1;
     static constexpr int rdw(int deg) { return C == 2 ? deg : (deg + 1) / 2; }
-    static constexpr int r_off(int li) {
+    // Group mode with 1024 threads and C = 2 (K = 16384 r1/2): 60 message
+    // dwords per thread do not fit a 1024-thread group's 128 VGPRs beside the
+    // working set, so the degree-3 layers keep their messages in LDS, laid
+    // out [dword][thread] (each thread reads only its own column: no bank
+    // conflicts, no barrier): 12 dwords x 1024 threads = 48 KiB beside L.
+    static constexpr bool LDS_R = !WAVE && THREADS == 1024 && C == 2;
+    static constexpr bool in_lds(int li) { return LDS_R && S.layer[li].deg == 3; }
+    static constexpr int r_off(int li) {  // offset in the VGPR array
         int o = 0;
-        for (int i = 0; i < li; ++i) o += rdw(S.layer[i].deg);
+        for (int i = 0; i < li; ++i) o += in_lds(i) ? 0 : rdw(S.layer[i].deg);
+        return o;
+    }
+    static constexpr int rl_off(int li) {  // offset in the LDS array (dwords per thread)
+        int o = 0;
+        for (int i = 0; i < li; ++i) o += in_lds(i) ? rdw(S.layer[i].deg) : 0;
         return o;
     }
     static constexpr int NR = r_off(NLAYERS);
+    static constexpr int NRL = rl_off(NLAYERS);
     static_assert(G >= 16 && G <= 1024 && (G & (G - 1)) == 0, "threads per codeword");
     static_assert(NTX % 4 == 0 && NP % 4 == 0 && N % 8 == 0, "channel layout");
 };
 
 // One layer of the thread's checks; Ls is the codeword's LLR array in LDS.
 template <class CF, int LI>
-__device__ __forceinline__ void run_layer(uint16_t *Ls, uint32_t (&R)[CF::NR], int t, float beta_e, float beta3) {
+__device__ __forceinline__ void run_layer(uint16_t *Ls, uint32_t (&R)[CF::NR], uint32_t *Rl, int t, float beta_e,
+                                          float beta3) {
     constexpr ar4ja::Layer Ly = CF::S.layer[LI];
-    constexpr int D = Ly.deg, R0 = CF::r_off(LI), Z = CF::Z, G = CF::G;
+    constexpr int D = Ly.deg, R0 = CF::r_off(LI), RL0 = CF::rl_off(LI), Z = CF::Z, G = CF::G;
     const float beta = D == 3 ? beta3 : beta_e;
     // Opaque copy of the thread index: keeps the shifted LDS addresses from
     // being hoisted out of the iteration loop (they would pin VGPRs).
@@ -129,7 +143,11 @@ __device__ __forceinline__ void run_layer(uint16_t *Ls, uint32_t (&R)[CF::NR], i
             const int xb = sh == 0 ? t + G : (xa ^ G);  // (t + G + sh) mod Z, G = Z/2
             ia[d] = base + xa;
             ib[d] = base + xb;
-            const uint32_t rp = R[R0 + d];
+            uint32_t rp;
+            if constexpr (CF::in_lds(LI))
+                rp = Rl[(RL0 + d) * G + t];
+            else
+                rp = R[R0 + d];
             qp[d] = cvt_pk_bf16(__fsub_rn(ld_bf(Ls + ia[d]), lo_f(rp)), __fsub_rn(ld_bf(Ls + ib[d]), hi_f(rp)));
             qa[d] = lo_f(qp[d]);
             qb[d] = hi_f(qp[d]);
@@ -146,7 +164,10 @@ __device__ __forceinline__ void run_layer(uint16_t *Ls, uint32_t (&R)[CF::NR], i
             const uint32_t ln = cvt_pk_bf16(__fadd_rn(qa[d], lo_f(rn)), __fadd_rn(qb[d], hi_f(rn)));
             Ls[ia[d]] = (uint16_t)ln;
             Ls[ib[d]] = (uint16_t)(ln >> 16);
-            R[R0 + d] = rn;
+            if constexpr (CF::in_lds(LI))
+                Rl[(RL0 + d) * G + t] = rn;
+            else
+                R[R0 + d] = rn;
         }
     } else {
         constexpr int DW = (D + 1) / 2;
@@ -188,12 +209,12 @@ __device__ __forceinline__ void run_layer(uint16_t *Ls, uint32_t (&R)[CF::NR], i
 }
 
 template <class CF, int... LI>
-__device__ __forceinline__ void run_iteration(uint16_t *Ls, uint32_t (&R)[CF::NR], int t, float beta_e, float beta3,
-                                              std::integer_sequence<int, LI...>) {
+__device__ __forceinline__ void run_iteration(uint16_t *Ls, uint32_t (&R)[CF::NR], uint32_t *Rl, int t, float beta_e,
+                                              float beta3, std::integer_sequence<int, LI...>) {
     if constexpr (CF::WAVE)
-        ((run_layer<CF, LI>(Ls, R, t, beta_e, beta3), wave_fence()), ...);
+        ((run_layer<CF, LI>(Ls, R, Rl, t, beta_e, beta3), wave_fence()), ...);
     else
-        ((run_layer<CF, LI>(Ls, R, t, beta_e, beta3), __syncthreads()), ...);
+        ((run_layer<CF, LI>(Ls, R, Rl, t, beta_e, beta3), __syncthreads()), ...);
 }
 
 // 1 if any of the thread's checks of layer LI is unsatisfied.
@@ -409,7 +430,7 @@ __global__ void __launch_bounds__(CF::THREADS) k_multi_wave(MultiParams P) {
             wave_fence();
         }
         if (!active) break;
-        run_iteration<CF>(Ls, R, t, P.beta, P.beta3, std::make_integer_sequence<int, NLAYERS>{});
+        run_iteration<CF>(Ls, R, nullptr, t, P.beta, P.beta3, std::make_integer_sequence<int, NLAYERS>{});
         it += 1;
         conv = syndrome_pass<CF, 0>(Ls, t, active);
         done = (conv | slot_any<CF>(__ballot(it >= P.max_iter))) & active;
@@ -429,6 +450,7 @@ __global__ void __launch_bounds__(CF::THREADS) k_multi_wave(MultiParams P) {
 template <class CF>
 __global__ void __launch_bounds__(CF::THREADS) k_multi_group(MultiParams P) {
     __shared__ __attribute__((aligned(16))) uint16_t Ls[CF::N];
+    __shared__ uint32_t Rl[CF::NRL > 0 ? CF::NRL * CF::G : 1];
     __shared__ uint32_t hist_s[HIST_LDS];
     __shared__ uint32_t flag_s[CF::WPW];
     __shared__ uint32_t cw_s, be_s;
@@ -463,11 +485,13 @@ __global__ void __launch_bounds__(CF::THREADS) k_multi_group(MultiParams P) {
         if (t == 0) be_s = 0u;
 #pragma unroll
         for (int i = 0; i < CF::NR; ++i) R[i] = 0u;
+#pragma unroll
+        for (int i = 0; i < CF::NRL; ++i) Rl[i * CF::G + t] = 0u;
         __syncthreads();
         int it = 0;
         bool ok = false;
         while (it < P.max_iter) {
-            run_iteration<CF>(Ls, R, t, P.beta, P.beta3, std::make_integer_sequence<int, NLAYERS>{});
+            run_iteration<CF>(Ls, R, Rl, t, P.beta, P.beta3, std::make_integer_sequence<int, NLAYERS>{});
             ++it;
             const bool f = wave_fails_from<CF, 0>(Ls, t);
             if (lane == 0) flag_s[wv] = f ? 1u : 0u;

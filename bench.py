@@ -13,9 +13,9 @@ all-zero codeword plus AWGN from the counter-based stream.
 run_simulation.py drives) on the reference's own .chinn.out matrix (AR4JA
 r1/2 K=1024 relabelled, rows in file order), 2.0 dB, 200 max iterations --
 the configuration of the reference's published 158 Msps scalar figure
-(README.md:108).  ``--workload floor`` is the error-floor configuration: 1e9 codewords at
-2.5 dB in 2^20-codeword launches with u64 counters (about 30 s on one
-MI355X).  ``--workload batch9`` measures the mixed-code extension: all nine
+(README.md:108).  ``--workload floor`` is the error-floor configuration: 1e9 codewords
+(``--blocks``) at 2.5 dB in 2^20-codeword launches with u64 counters (about 30 s
+on one MI355X), recording the ids of failing codewords.  ``--workload batch9`` measures the mixed-code extension: all nine
 AR4JA codes (K = 1024, 4096, 16384 x rate 1/2, 2/3, 4/5), equal codewords per
 code, each at its own Eb/N0 in [1, 3] dB, one captured HIP graph per step
 holding every code's persistent kernel.
@@ -258,6 +258,7 @@ def bench_scalar(args) -> None:
 def bench_floor(args) -> None:
     from ldpc_mi355x import decoder
     eng = decoder.Engine(1024, "1/2", max_iter=MAX_ITER, beta=BETA)
+    eng.set_capture(64)  # ids of the failing codewords (replayable: seed + id regenerate the channel)
     n, acc, first = args.blocks, None, 0
     t0 = time.perf_counter()
     while (acc.blocks if acc else 0) < n:
@@ -269,13 +270,15 @@ def bench_floor(args) -> None:
     wall = time.perf_counter() - t0
     lo, hi = poisson_ci95(acc.frame_errors)
     latest = max([i for i in range(1, MAX_ITER + 1) if acc.hist[i] > 0] or [0])
-    out = {"metric": "Gsps (channel sym/s) 1 GPU, AR4JA r1/2 K=1024 @2.5dB error floor, 1e9 codewords",
+    ids, _ = eng.captures()
+    out = {"metric": f"Gsps (channel sym/s) 1 GPU, AR4JA r1/2 K=1024 @2.5dB error floor, {acc.blocks:.3g} codewords",
            "value": acc.blocks * N_TX / wall / 1e9, "unit": "Gsps", "n_gpus": 1, "blocks": acc.blocks,
            "launches": acc.launches, "frame_errors": acc.frame_errors, "bit_errors": acc.bit_errors,
            "fer": acc.fer, "fer_ci95": [lo / acc.blocks, hi / acc.blocks], "mean_iterations": acc.mean_iterations,
            "latest_converged_iteration": latest, "wall_s": wall, "kernel_gsps": acc.kernel_gsps,
            "higher_is_better": True, "dtype": "bf16", "data": "synthetic (on-device Philox/Box-Muller)",
-           "reference_published_fer": 1.53e-9}
+           "reference_published_fer": 1.53e-9, "seed": 0x5EED2026 ^ 0x2500,
+           "failing_codeword_ids": [int(i) for i in ids]}
     print(json.dumps(out), flush=True)
     eng.close()
 

@@ -97,3 +97,26 @@ def test_cli_trap_writes_dumps(tmp_path):
     init = dumps.read_packed4(str(tmp_path / dumps.INITIAL))
     with decoder.Engine(1024, "1/2", max_iter=20) as eng:
         np.testing.assert_array_equal(dumps.read_packed4(str(tmp_path / dumps.FINAL)), dumps.replay(eng, init)["final"])
+
+
+def test_trap_report_cli(tmp_path, capsys):
+    """python -m ldpc_mi355x.dumps on captured ids: dumps written, the failure
+    reproduced (not converged, 20 iterations) and its (a, b) signature equal to
+    the one computed from the replayed final state."""
+    with decoder.Engine(1024, "1/2", max_iter=20) as eng:
+        eng.set_capture(4)
+        eng.run_point(1.0, 400, seed=6)
+        ids, _ = eng.captures()
+        assert len(ids) >= 2
+        rep = dumps.trap_dumps(eng, 6, 1.0, int(ids[1]), str(tmp_path / "direct"))
+    exp = dumps.trapping_set(codes.ar4ja(1024, "1/2"), rep["final"][rep["failing_cw"]])
+    assert dumps.main(["--seed", "6", "--ebn0", "1.0", "--max-iter", "20", "--id", str(ids[0]), "--id", str(ids[1]),
+                       "--out", str(tmp_path / "cli")]) == 0
+    import json
+    lines = [json.loads(x) for x in capsys.readouterr().out.splitlines() if x.startswith("{")]
+    assert [ln["id"] for ln in lines] == [int(ids[0]), int(ids[1])]
+    for ln in lines:
+        assert not ln["converged"] and ln["iterations"] == 20 and ln["a"] >= 1 and ln["b"] >= 1
+        assert os.path.getsize(os.path.join(ln["dir"], dumps.FINAL)) == 20480
+    assert (lines[1]["a"], lines[1]["b"], lines[1]["info_bits_in_error"]) == (exp["a"], exp["b"], exp["info_bits"])
+    assert lines[1]["bit_errors"] == rep["bit_errors"][rep["failing_cw"]]

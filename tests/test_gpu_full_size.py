@@ -1,13 +1,17 @@
 """Parity at the plain configurations' own sizes (AR4JA r1/2 K=1024, 200
 iterations): 1e5 codewords per point at 1.0 / 1.5 / 2.0 dB through the fused
 Monte Carlo kernel, counters and the whole iteration histogram equal to the
-oracle's; and a bit-exact decode of a 1e5-codeword LLR dump at 2.5 dB (final
-LLRs, iterations, convergence)."""
+oracle's; a bit-exact decode of a 1e5-codeword LLR dump at 2.5 dB (final
+LLRs, iterations, convergence); and the same 1e5-codeword Monte Carlo check
+for Profile R (the scalar_5w arithmetic run_simulation.py drives) on the
+reference's .chinn.out."""
+import os
+
 import numpy as np
 import pytest
 
 from conftest import channel_params
-from ldpc_mi355x import decoder
+from ldpc_mi355x import codes, decoder
 
 pytestmark = pytest.mark.gpu
 
@@ -37,3 +41,18 @@ def test_decode_1e5_codeword_dump_at_2p5db(oracle, code12):
     np.testing.assert_array_equal(ok, e_ok)
     np.testing.assert_array_equal(it, e_it)
     np.testing.assert_array_equal(out, e_out)
+
+
+@pytest.mark.parametrize("ebn0", [1.5, 2.0])
+def test_profile_r_monte_carlo_1e5_codewords_matches_oracle(oracle, ebn0):
+    _, n, rows = codes.read_chinn_out(os.path.join(os.path.dirname(__file__), "golden",
+                                                   "AR4JA_r45_4c_128c_r12.chinn.out"))
+    print(f"Profile R 1e5-codeword Monte Carlo at {ebn0} dB", flush=True)
+    with decoder.Engine(profile="R", matrix=(rows, n), n_punct=512, max_iter=200) as eng:
+        got = eng.run_point(ebn0, N_CW, seed=0x5EED2026, chunk_cw=1 << 20)
+        mu, sg = eng.channel_params(ebn0)
+    exp = oracle.run_point_r(n, rows, 2048, 0x5EED2026, 0, N_CW, mu, sg, 0.0, 200)
+    assert (got.blocks, got.frame_errors, got.bit_errors, got.iter_sum) == \
+        (exp["blocks"], exp["frame_errors"], exp["bit_errors"], exp["iter_sum"])
+    np.testing.assert_array_equal(got.hist, exp["hist"])
+    print(f"  FER {got.fer:.3e}  BER {got.ber:.3e}  mean iterations {got.mean_iterations:.3f}", flush=True)

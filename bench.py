@@ -82,6 +82,20 @@ def ops_per_cwi(k: int, rate: str) -> int:
     return m * (ops_per_check(3) + 2 * ops_per_check(DEGREE[rate]))
 
 
+def fer_match(gpu_fe: int, gpu_blocks: int, cpu):
+    """FER consistency of the GPU run and the CPU baseline's sample: two-sided
+    exact test of equal Poisson rates (conditional binomial on the total
+    count).  None when the baseline reports no counts."""
+    if not cpu or cpu.get("blocks") is None or cpu.get("frame_errors") is None:
+        return None
+    from scipy.stats import binomtest
+    n_cpu, k_cpu = int(cpu["blocks"]), int(cpu["frame_errors"])
+    tot = gpu_fe + k_cpu
+    p = 1.0 if tot == 0 else binomtest(k_cpu, tot, n_cpu / (n_cpu + gpu_blocks)).pvalue
+    return {"gpu_frame_errors": gpu_fe, "gpu_blocks": gpu_blocks, "cpu_frame_errors": k_cpu, "cpu_blocks": n_cpu,
+            "p_value": float(p), "consistent_95": bool(p >= 0.05)}
+
+
 def poisson_ci95(k: int):
     """Exact (Garwood) 95% interval of a Poisson count."""
     from scipy.stats import chi2
@@ -112,8 +126,11 @@ def cpu_reference_baseline(seconds: float):
     if not m:
         return None
     return {"value": float(m.group(1)) / 1e3, "unit": "Gsps", "cores": threads, "kind": "reference",
+            "blocks": int(tb.group(1)) if tb else None, "frame_errors": int(fe.group(1)) if fe else None,
+            "beta": 1.0, "mean_iterations": float(it.group(1)) if it else None,
             "sample": f"reference golden stepper (tests/test_circulant_dma_stepper.cpp, built from source) "
-                      f"--sfpi --monte-carlo {EBN0_DB} dB, {tb.group(1)} codewords, 200 max iter, "
+                      f"--sfpi --monte-carlo {EBN0_DB} dB (its Monte Carlo mode decodes with beta 1), "
+                      f"{tb.group(1)} codewords, 200 max iter, "
                       f"{threads} OpenMP threads; mean iterations {it.group(1) if it else '?'}, "
                       f"FE {fe.group(1) if fe else '?'}"}
 
@@ -127,14 +144,15 @@ def cpu_oracle_baseline(seconds: float):
     g = 10 ** (EBN0_DB / 10)
     mu, sg = 4 * 0.5 * g, math.sqrt(8 * 0.5 * g)
     threads = O.num_threads()
-    blocks, t0, first = 0, time.time(), 0
+    blocks, fe, t0, first = 0, 0, time.time(), 0
     while time.time() - t0 < seconds:
         n = 512 * threads
-        O.run_point_t(code, 0xC0DE, first, n, mu, sg, MAX_ITER, BETA, threads)
+        fe += O.run_point_t(code, 0xC0DE, first, n, mu, sg, MAX_ITER, BETA, threads)["frame_errors"]
         blocks += n
         first += n
     dt = time.time() - t0
     return {"value": blocks * N_TX / dt / 1e9, "unit": "Gsps", "cores": threads, "kind": "port",
+            "blocks": blocks, "frame_errors": fe,
             "sample": f"CPU oracle (oracle/ldpc_oracle.c, Profile T), {blocks} codewords at {EBN0_DB} dB, "
                       f"200 max iter, {threads} OpenMP threads"}
 
@@ -194,13 +212,15 @@ def cpu_oracle_scalar(rows, n, seconds: float):
     threads = O.num_threads()
     g = 10 ** (EBN0_DB / 10)
     mu, sg = 4 * 0.5 * g, math.sqrt(8 * 0.5 * g)
-    blocks, t0 = 0, time.time()
+    blocks, fe, t0 = 0, 0, time.time()
     while time.time() - t0 < seconds:
         nb = 256 * threads
-        O.run_point_r(n, rows, N_TX, 0xC0DE, blocks, nb, mu, sg, max_iter=MAX_ITER, threads=threads)
+        fe += O.run_point_r(n, rows, N_TX, 0xC0DE, blocks, nb, mu, sg, max_iter=MAX_ITER,
+                            threads=threads)["frame_errors"]
         blocks += nb
     dt = time.time() - t0
     return {"value": blocks * N_TX / dt / 1e9, "unit": "Gsps", "cores": threads, "kind": "port",
+            "blocks": blocks, "frame_errors": fe,
             "sample": f"CPU oracle (oracle/ldpc_oracle.c, Profile R) on the .chinn.out, {blocks} codewords at "
                       f"{EBN0_DB} dB, 200 max iter, {threads} OpenMP threads"}
 
@@ -250,6 +270,7 @@ def bench_scalar(args) -> None:
         "kernel_gsps": acc.blocks * N_TX / acc.kernel_s / 1e9,
         "reference_published_gsps": 0.158,
         "speedup_vs_cpu": (value / cpu["value"]) if cpu and cpu.get("value") else None,
+        "fer_match": fer_match(acc.frame_errors, acc.blocks, cpu),
     }
     print(json.dumps(out), flush=True)
     eng.close()
@@ -406,6 +427,22 @@ def main():
         except Exception as e:  # a baseline failure must not hide the GPU number
             cpu = {"value": None, "unit": "Gsps", "cores": os.cpu_count(), "kind": "reference",
                    "sample": f"failed: {e}"}
+    match = fer_match(acc.frame_errors, acc.blocks, cpu)
+    if cpu and cpu.get("beta", BETA) != BETA:
+        # The reference stepper's --monte-carlo mode decodes with beta = 1
+        # (tests/test_circulant_dma_stepper.cpp:708, 1029): match its FER against a
+        # GPU sample at that beta, and report the GPU rate at that beta too.
+        eng1 = decoder.Engine(1024, "1/2", max_iter=MAX_ITER, beta=cpu["beta"], device=0)
+        n1 = 8 * CW_PER_STEP
+        t1 = time.perf_counter()
+        r1 = eng1.run_point(EBN0_DB, n1, seed=seed ^ 0xB1, chunk_cw=CW_PER_STEP)
+        eng1.sync()
+        g1 = r1.blocks * N_TX / (time.perf_counter() - t1) / 1e9
+        eng1.close()
+        match = fer_match(r1.frame_errors, r1.blocks, cpu)
+        if match:
+            match.update({"gpu_beta": cpu["beta"], "gpu_gsps_at_beta": g1, "gpu_mean_iterations": r1.mean_iterations,
+                          "speedup_vs_cpu_at_beta": g1 / cpu["value"] if cpu.get("value") else None})
 
     out = {
         "metric": "Gsps (channel sym/s) 1 GPU, AR4JA r1/2 K=1024 @2.0dB; FER-matched speedup",
@@ -438,6 +475,7 @@ def main():
         "ber": acc.ber, "mean_iterations": acc.mean_iterations, "blocks": acc.blocks,
         "kernel_gsps": acc.blocks * N_TX / kernel_s / 1e9,
         "speedup_vs_cpu": (value / cpu["value"]) if cpu and cpu.get("value") else None,
+        "fer_match": match,
         "reference_published_gsps": 5.2873,
     }
     print(json.dumps(out), flush=True)

@@ -1,0 +1,38 @@
+"""Host-side arithmetic of bench.py: the algorithmic work model and the
+FER-matched comparison with the CPU baseline (no GPU needed)."""
+import os
+import sys
+
+import pytest
+
+sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+import bench  # noqa: E402
+
+
+def test_work_model_matches_the_degree_table():
+    # 239 lane-ops per degree-6 check and 86 per degree-3 check (18d - 1 + 11 min* evaluations)
+    assert bench.ops_per_check(6) == 239 and bench.ops_per_check(3) == 86
+    assert bench.ops_per_cwi(1024, "1/2") == bench.OPS_PER_CWI == 1024 * 239 + 512 * 86
+    # every code: M checks of degree 3 plus 2M of the rate's degree
+    assert bench.ops_per_cwi(16384, "4/5") == 16 * 128 * (86 + 2 * bench.ops_per_check(18))
+
+
+def test_fer_match_is_an_exact_two_sample_poisson_test():
+    scipy = pytest.importorskip("scipy")  # noqa: F841
+    same = bench.fer_match(57, 20_971_520, {"blocks": 368_640, "frame_errors": 1})
+    assert same["consistent_95"] and same["p_value"] > 0.05
+    far = bench.fer_match(57, 20_971_520, {"blocks": 368_640, "frame_errors": 30})
+    assert not far["consistent_95"] and far["p_value"] < 1e-20
+    assert bench.fer_match(0, 100, {"blocks": 10, "frame_errors": 0})["p_value"] == 1.0
+    assert bench.fer_match(3, 100, None) is None
+    assert bench.fer_match(3, 100, {"blocks": None, "frame_errors": None}) is None
+    # symmetric in which side holds the errors when the sample sizes are equal
+    a = bench.fer_match(10, 1000, {"blocks": 1000, "frame_errors": 2})["p_value"]
+    b = bench.fer_match(2, 1000, {"blocks": 1000, "frame_errors": 10})["p_value"]
+    assert abs(a - b) < 1e-12
+
+
+def test_gpus_above_one_exit_with_a_message():
+    import subprocess
+    p = subprocess.run([sys.executable, bench.__file__, "--gpus", "2"], capture_output=True, text=True, timeout=60)
+    assert p.returncode == 2 and "one GPU only" in p.stderr and p.stdout == ""

@@ -37,6 +37,15 @@ struct ldpc_batch {
     std::vector<int> grid;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // The codes' kernels run on side streams forked from `stream` inside the
+    // graph: one stream for the codes decoded one codeword per workgroup
+    // (group mode, whole-CU workgroups), one for the codes with several
+    // codewords per workgroup (wave mode), so the short wave-mode kernels and
+    // their tails (codewords still iterating) overlap the long ones.
+    std::vector<hipStream_t> side;
+    std::vector<int> lane;  // side stream of each code
+    hipEvent_t fork = nullptr;
+    std::vector<hipEvent_t> join;
     uint32_t *d_queue = nullptr;               // [n_codes]
     unsigned long long *d_counters = nullptr;  // [4 n_codes]
     unsigned long long *d_hist = nullptr;      // [n_codes nh]
@@ -61,7 +70,9 @@ namespace {
 
 // One round of the batch as a graph: parameters in, counters cleared, every
 // code's persistent kernel, counters and histograms out.  Counters are
-// cleared with fill kernels, not memset nodes.
+// cleared with fill kernels, not memset nodes.  The kernels are independent
+// (own queue, counters, histogram and codeword ids), so they hang off a
+// fork/join of side streams and may run concurrently.
 int capture_graph(ldpc_batch *b) {
     const uint32_t n = b->n_codes;
     B_TRY(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
@@ -72,6 +83,9 @@ int capture_graph(ldpc_batch *b) {
         if ((e = ldpc::launch_fill_u32(b->d_queue, n, 0u, b->stream))) return e;
         if ((e = ldpc::launch_fill_u64(b->d_counters, 4 * (size_t)n, 0ull, b->stream))) return e;
         if ((e = ldpc::launch_fill_u64(b->d_hist, (size_t)n * b->nh, 0ull, b->stream))) return e;
+        if ((e = hipEventRecord(b->fork, b->stream))) return e;
+        for (hipStream_t s : b->side)
+            if ((e = hipStreamWaitEvent(s, b->fork, 0))) return e;
         for (uint32_t i = 0; i < n; ++i) {
             ldpc::MultiParams p{};
             p.queue = b->d_queue + i;
@@ -81,7 +95,11 @@ int capture_graph(ldpc_batch *b) {
             p.max_iter = (int)b->cfg.max_iter;
             p.beta = b->cfg.beta;
             p.beta3 = b->cfg.beta < 1.0f ? b->cfg.beta : 1.0f;  // stepper:300-304
-            if ((e = b->mv[i]->launch(p, b->grid[i], b->stream))) return e;
+            if ((e = b->mv[i]->launch(p, b->grid[i], b->side[b->lane[i]]))) return e;
+        }
+        for (size_t k = 0; k < b->side.size(); ++k) {
+            if ((e = hipEventRecord(b->join[k], b->side[k]))) return e;
+            if ((e = hipStreamWaitEvent(b->stream, b->join[k], 0))) return e;
         }
         if ((e = hipMemcpyAsync(b->h_counters, b->d_counters, 4 * n * sizeof(unsigned long long),
                                 hipMemcpyDeviceToHost, b->stream)))
@@ -178,6 +196,11 @@ void ldpc_batch_destroy(ldpc_batch *b) {
         if (p) (void)hipHostFree(p);
     if (b->ev0) (void)hipEventDestroy(b->ev0);
     if (b->ev1) (void)hipEventDestroy(b->ev1);
+    if (b->fork) (void)hipEventDestroy(b->fork);
+    for (hipEvent_t e : b->join)
+        if (e) (void)hipEventDestroy(e);
+    for (hipStream_t s : b->side)
+        if (s) (void)hipStreamDestroy(s);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
 }
@@ -241,6 +264,17 @@ int ldpc_batch_create(const ldpc_code_desc *codes, const uint32_t *variants, uin
     B_STEP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
     B_STEP(hipEventCreate(&b->ev0));
     B_STEP(hipEventCreate(&b->ev1));
+    B_STEP(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
+    b->lane.resize(n_codes);
+    bool any_group = false, any_wave = false;
+    for (uint32_t i = 0; i < n_codes; ++i) (mv[i]->cw_per_wg > 1 ? any_wave : any_group) = true;
+    for (uint32_t i = 0; i < n_codes; ++i) b->lane[i] = (any_group && any_wave && mv[i]->cw_per_wg > 1) ? 1 : 0;
+    b->side.assign(any_group && any_wave ? 2 : 1, nullptr);  // default priority, like every stream here
+    b->join.assign(b->side.size(), nullptr);
+    for (size_t k = 0; k < b->side.size(); ++k) {
+        B_STEP(hipStreamCreateWithFlags(&b->side[k], hipStreamNonBlocking));
+        B_STEP(hipEventCreateWithFlags(&b->join[k], hipEventDisableTiming));
+    }
     B_STEP(hipMalloc(&b->d_queue, n_codes * sizeof(uint32_t)));
     B_STEP(hipMalloc(&b->d_counters, 4 * n_codes * sizeof(unsigned long long)));
     B_STEP(hipMalloc(&b->d_hist, (size_t)n_codes * b->nh * sizeof(unsigned long long)));

@@ -3,10 +3,10 @@
 TEST INFRASTRUCTURE ONLY: imported by tests/ and by bench.py's cpu_baseline
 leg, never by the product package.
 
-The reference has no differentiable decoder (SURVEY: "no reference
-counterpart"); this file states the arithmetic the HIP kernels implement, in
-numpy, so that the GPU forward can be compared bit for bit (float32, every
-op separately rounded, as numpy does) and the GPU gradient against central
+The reference has no differentiable decoder; this file states the
+arithmetic the HIP kernels implement, in numpy, so that the GPU forward can
+be compared bit for bit (float32, every op separately rounded, as numpy
+does) and the GPU gradient against central
 finite differences of the same function evaluated in float64.
 
 Decoder (per layer of the AR4JA layered schedule, checks in parallel):

@@ -1,7 +1,7 @@
-"""The learned decoder's training and judging tools, end to end on tiny sizes:
+"""The learned decoder's training and evaluation tools, end to end on tiny sizes:
 train_dmin.py writes parameters; a second run resumes from them
 (--init-from, checked with lr 0 so the output must equal the input) and a
-parameter file of another shape is refused; eval_dmin.py judges them through
+parameter file of another shape is refused; eval_dmin.py evaluates them through
 Profile D with early exit, and through the training forward at an iteration
 count other than the trained one (--forward-T)."""
 import json

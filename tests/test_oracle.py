@@ -230,7 +230,8 @@ def test_run_point_equals_decode_of_generated(oracle, code12):
 
 
 def test_mean_iterations_near_golden(oracle, code12):
-    """Mean iterations at 2.0 dB are about 6.8 (golden, 40k blocks)."""
+    """Mean iterations at 2.0 dB are about 6.8: the reference golden stepper's Monte Carlo
+    reports 6.81 on 40k codewords at that point."""
     mu, sg = channel_params(2.0)
     r = oracle.run_point_t(code12, 1, 0, 3000, mu, sg, 200, 0.9375)
     assert 6.4 < r["iter_sum"] / r["blocks"] < 7.3

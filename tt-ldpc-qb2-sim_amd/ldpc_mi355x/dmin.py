@@ -14,8 +14,8 @@ optionally with the Approx-Min* correction of the reference's golden tree
 are hand-written HIP kernels (csrc/dmin_torch.hip) behind a
 ``torch.autograd.Function``; activations are recomputed from (L, R)
 checkpoints taken every ``ckpt_every`` layer steps.  The reference has no
-counterpart of this decoder; it is judged by the plain Monte Carlo FER of
-the trained parameters (libldpc Profile D).
+counterpart of this decoder; it is evaluated by the plain Monte Carlo FER
+of the trained parameters (libldpc Profile D).
 """
 from __future__ import annotations
 

@@ -1,5 +1,5 @@
 #!/usr/bin/env python3
-"""Judge a trained layered min-sum decoder by plain Monte Carlo on one GPU.
+"""Evaluate a trained layered min-sum decoder by plain Monte Carlo on one GPU.
 
 Runs libldpc Profile D (the trained parameters, syndrome early exit) at each
 Eb/N0 for the given iteration caps, and Profile T (the golden Approx-Min*)

@@ -10,7 +10,6 @@ import os
 import numpy as np
 import pytest
 
-from conftest import channel_params
 from ldpc_mi355x import _lib as L
 from ldpc_mi355x import codes, decoder
 

@@ -168,4 +168,4 @@ def test_training_steps_run():
         after = float(D.soft_bce(model(llr), model.n_tx))
     print("loss", before, "->", after, [(l.step, l.loss) for l in log], flush=True)
     assert np.isfinite(after) and all(np.isfinite(l.loss) for l in log)
-    assert float((model.alpha - 1).abs().max()) > 0 and float(model.beta.abs().max()) > 0
+    assert float((model.alpha.detach() - 1).abs().max()) > 0 and float(model.beta.detach().abs().max()) > 0

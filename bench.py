@@ -207,6 +207,36 @@ def cpu_oracle_batch9(seconds: float):
 SCALAR_MATRIX = os.path.join(ROOT, "tests", "golden", "AR4JA_r45_4c_128c_r12.chinn.out")
 
 
+def cpu_reference_scalar(seconds: float):
+    """The reference's scalar_5w worker (run_ldpc_worker, engines/scalar_5w/
+    ldpc_decoder_core.h:214, built from source into oracle/_ref by
+    oracle/Makefile.ref) on all host threads, one worker stream per thread,
+    with its own xoshiro128+ / polar-Marsaglia channel."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "scalar_driver")
+    if not os.path.exists(exe):
+        return None
+    threads = os.cpu_count() or 1
+
+    def run(per_thread):
+        p = subprocess.run([exe, "--monte-carlo", SCALAR_MATRIX, str(EBN0_DB), str(2560 - N_TX), str(MAX_ITER),
+                            str(per_thread), str(threads), "133742"], capture_output=True, text=True, timeout=900)
+        return json.loads(p.stdout.strip().splitlines()[-1]) if p.returncode == 0 else None
+
+    probe = run(200)
+    if not probe:
+        return None
+    per = max(200, int(200 * seconds / max(probe["seconds"], 1e-3)) // 2 * 2)
+    r = run(per)
+    if not r:
+        return None
+    return {"value": r["msps"] / 1e3, "unit": "Gsps", "cores": threads, "kind": "reference",
+            "blocks": r["codewords"], "frame_errors": r["frame_errors"],
+            "sample": f"reference scalar_5w worker (run_ldpc_worker, engines/scalar_5w/ldpc_decoder_core.h, built "
+                      f"from source) on the .chinn.out, {r['codewords']} codewords at {EBN0_DB} dB, 200 max iter, "
+                      f"one worker per thread on {threads} threads, its own xoshiro128+ channel; "
+                      f"FE {r['frame_errors']}"}
+
+
 def cpu_oracle_scalar(rows, n, seconds: float):
     import oracle as O
     threads = O.num_threads()
@@ -248,7 +278,7 @@ def bench_scalar(args) -> None:
     cpu = None
     if not args.no_cpu:
         try:
-            cpu = cpu_oracle_scalar(rows, n, args.cpu_seconds)
+            cpu = cpu_reference_scalar(args.cpu_seconds) or cpu_oracle_scalar(rows, n, args.cpu_seconds)
         except Exception as e:
             cpu = {"value": None, "unit": "Gsps", "cores": os.cpu_count(), "kind": "port", "sample": f"failed: {e}"}
     fe_lo, fe_hi = poisson_ci95(acc.frame_errors)

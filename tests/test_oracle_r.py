@@ -100,3 +100,37 @@ def test_level_parallel_decode_equals_sequential(chinn, oracle):
     par, it2, _ = oracle.decode_r(N, [rows[i] for i in order], llr, 40)
     np.testing.assert_array_equal(seq, par)
     np.testing.assert_array_equal(it1, it2)
+
+
+def test_reference_worker_monte_carlo_mode_matches_fixture_mode(tmp_path):
+    """The CPU baseline mode of oracle/_ref/scalar_driver runs the same worker
+    call as the fixture mode: one thread, two codewords, same seed -> the same
+    frame and bit errors as the fixture mode's stats; and at 1.0 dB the
+    reference worker's FER agrees with the oracle's Profile R within sampling
+    error (different channel streams)."""
+    import json
+    import subprocess
+    exe = os.path.join(os.path.dirname(GOLDEN_DIR), "..", "oracle", "_ref", "scalar_driver")
+    if not os.path.exists(exe):
+        pytest.skip("oracle/_ref/scalar_driver not built (needs the reference sources)")
+    for ebn0, seed in ((0.5, 7), (1.0, 11)):
+        out = str(tmp_path / f"f{seed}.bin")
+        subprocess.run([exe, MATRIX, str(ebn0), "512", "30", str(seed), "0", "0", "0", out], check=True, timeout=60)
+        stats = np.fromfile(out, dtype="<u4")[-4:]
+        p = subprocess.run([exe, "--monte-carlo", MATRIX, str(ebn0), "512", "30", "2", "1", str(seed)],
+                           capture_output=True, text=True, check=True, timeout=60)
+        r = json.loads(p.stdout)
+        assert (r["codewords"], r["bit_errors"], r["frame_errors"]) == (2, int(stats[0]), int(stats[1]))
+    import math
+
+    from scipy.stats import binomtest
+    import oracle as O
+    p = subprocess.run([exe, "--monte-carlo", MATRIX, "1.0", "512", "30", "500", "8", "5"],
+                       capture_output=True, text=True, check=True, timeout=120)
+    r = json.loads(p.stdout)
+    M, N, rows = codes.read_chinn_out(MATRIX)
+    g = 10 ** 0.1
+    o = O.run_point_r(N, rows, 2048, 5, 0, 4000, 2 * g, math.sqrt(4 * g), 0.0, 30)
+    k1, n1, k2, n2 = r["frame_errors"], r["codewords"], o["frame_errors"], o["blocks"]
+    assert k1 > 50 and k2 > 50
+    assert binomtest(k1, k1 + k2, n1 / (n1 + n2)).pvalue > 1e-3

@@ -98,13 +98,31 @@ struct Cfg {
     static constexpr int THREADS = 64 * WPW;
     static constexpr int CW_PER_WG = WAVE ? SLOTS * WPW : 1;
     static constexpr int rdw(int deg) { return C == 2 ? deg : (deg + 1) / 2; }
-    // Group mode with 1024 threads and C = 2 (K = 16384 r1/2): 60 message
-    // dwords per thread do not fit a 1024-thread group's 128 VGPRs beside the
-    // working set, so the degree-3 layers keep their messages in LDS, laid
-    // out [dword][thread] (each thread reads only its own column: no bank
-    // conflicts, no barrier): 12 dwords x 1024 threads = 48 KiB beside L.
-    static constexpr bool LDS_R = !WAVE && THREADS == 1024 && C == 2;
-    static constexpr bool in_lds(int li) { return LDS_R && S.layer[li].deg == 3; }
+    // Messages of some layers live in LDS instead of VGPRs, laid out
+    // [dword][thread of the workgroup] (each thread reads only its own
+    // column: no bank conflicts, no barrier):
+    //  * group mode with 1024 threads and C = 2 (K = 16384 r1/2): 60 message
+    //    dwords per thread do not fit a 1024-thread group's 128 VGPRs beside
+    //    the working set; the degree-3 layers go to LDS (12 dwords x 1024
+    //    threads = 48 KiB beside the 80 KiB of L);
+    //  * rate 4/5, C = 1, group mode with at most 256 threads (K = 4096):
+    //    80 message dwords and the degree-18 working set take 177 VGPRs,
+    //    2 waves per SIMD; the degree-3 layers and the first two degree-18
+    //    layers go to LDS (26 dwords per thread): 161 VGPRs, 3 waves per SIMD,
+    //    6% faster at 3.0 dB.  (K = 1024 r4/5 in wave mode reaches 3 waves only
+    //    with a forced 168-VGPR budget, which cost more than it gained at
+    //    3.5 dB, so it keeps every message in VGPRs.)
+    static constexpr bool LDS_R12 = !WAVE && THREADS == 1024 && C == 2;
+    static constexpr bool LDS_R45 = LDPC_RID == 2 && C == 1 && !WAVE && THREADS <= 256;
+    static constexpr int deg18_before(int li) {
+        int n = 0;
+        for (int i = 0; i < li; ++i) n += S.layer[i].deg == 18 ? 1 : 0;
+        return n;
+    }
+    static constexpr bool in_lds(int li) {
+        return (LDS_R12 && S.layer[li].deg == 3) ||
+               (LDS_R45 && (S.layer[li].deg == 3 || (S.layer[li].deg == 18 && deg18_before(li) < 2)));
+    }
     static constexpr int r_off(int li) {  // offset in the VGPR array
         int o = 0;
         for (int i = 0; i < li; ++i) o += in_lds(i) ? 0 : rdw(S.layer[i].deg);
@@ -122,6 +140,8 @@ struct Cfg {
 };
 
 // One layer of the thread's checks; Ls is the codeword's LLR array in LDS.
+// Rl: the thread's column of the LDS message array (element k at
+// Rl[k * THREADS]); unused when the configuration keeps every message in VGPRs.
 template <class CF, int LI>
 __device__ __forceinline__ void run_layer(uint16_t *Ls, uint32_t (&R)[CF::NR], uint32_t *Rl, int t, float beta_e,
                                           float beta3) {
@@ -145,7 +165,7 @@ __device__ __forceinline__ void run_layer(uint16_t *Ls, uint32_t (&R)[CF::NR], u
             ib[d] = base + xb;
             uint32_t rp;
             if constexpr (CF::in_lds(LI))
-                rp = Rl[(RL0 + d) * G + t];
+                rp = Rl[(RL0 + d) * CF::THREADS];
             else
                 rp = R[R0 + d];
             qp[d] = cvt_pk_bf16(__fsub_rn(ld_bf(Ls + ia[d]), lo_f(rp)), __fsub_rn(ld_bf(Ls + ib[d]), hi_f(rp)));
@@ -165,7 +185,7 @@ __device__ __forceinline__ void run_layer(uint16_t *Ls, uint32_t (&R)[CF::NR], u
             Ls[ia[d]] = (uint16_t)ln;
             Ls[ib[d]] = (uint16_t)(ln >> 16);
             if constexpr (CF::in_lds(LI))
-                Rl[(RL0 + d) * G + t] = rn;
+                Rl[(RL0 + d) * CF::THREADS] = rn;
             else
                 R[R0 + d] = rn;
         }
@@ -178,7 +198,11 @@ __device__ __forceinline__ void run_layer(uint16_t *Ls, uint32_t (&R)[CF::NR], u
 #pragma unroll
         for (int w = 0; w < DW; ++w) {
             const int d0 = 2 * w, d1 = 2 * w + 1;
-            const uint32_t rp = R[R0 + w];
+            uint32_t rp;
+            if constexpr (CF::in_lds(LI))
+                rp = Rl[(RL0 + w) * CF::THREADS];
+            else
+                rp = R[R0 + w];
             ia[d0] = Ly.slot[d0].base + ((t + Ly.slot[d0].shift) & (Z - 1));
             const float a = __fsub_rn(ld_bf(Ls + ia[d0]), lo_f(rp));
             float b = 0.0f;
@@ -203,7 +227,10 @@ __device__ __forceinline__ void run_layer(uint16_t *Ls, uint32_t (&R)[CF::NR], u
             const uint32_t ln = cvt_pk_bf16(__fadd_rn(q[d0], lo_f(rn)), __fadd_rn(q[d1], hi_f(rn)));
             Ls[ia[d0]] = (uint16_t)ln;
             if (d1 < D) Ls[ia[d1]] = (uint16_t)(ln >> 16);
-            R[R0 + w] = rn;
+            if constexpr (CF::in_lds(LI))
+                Rl[(RL0 + w) * CF::THREADS] = rn;
+            else
+                R[R0 + w] = rn;
         }
     }
 }
@@ -362,11 +389,13 @@ __device__ __forceinline__ void hist_flush(const uint32_t *hist_s, const MultiPa
 template <class CF>
 __global__ void __launch_bounds__(CF::THREADS) k_multi_wave(MultiParams P) {
     __shared__ __attribute__((aligned(16))) uint16_t smem[CF::CW_PER_WG * CF::N];
+    __shared__ uint32_t Rl_s[CF::NRL > 0 ? CF::NRL * CF::THREADS : 1];
     __shared__ uint32_t hist_s[HIST_LDS];
     const Run run = load_run(P);
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int slot = lane / CF::G, t = lane % CF::G;
     uint16_t *Ls = smem + (wv * CF::SLOTS + slot) * CF::N;
+    uint32_t *Rl = Rl_s + threadIdx.x;
     for (int i = threadIdx.x; i < HIST_LDS; i += CF::THREADS) hist_s[i] = 0u;
     __syncthreads();
 
@@ -426,11 +455,15 @@ __global__ void __launch_bounds__(CF::THREADS) k_multi_wave(MultiParams P) {
             }
 #pragma unroll
             for (int i = 0; i < CF::NR; ++i) R[i] = mine ? 0u : R[i];  // bf16 +0: iteration 0 has q = L
+            if (mine) {
+#pragma unroll
+                for (int i = 0; i < CF::NRL; ++i) Rl[i * CF::THREADS] = 0u;
+            }
             active = slot_any<CF>(__ballot(cw < run.n_cw));
             wave_fence();
         }
         if (!active) break;
-        run_iteration<CF>(Ls, R, nullptr, t, P.beta, P.beta3, std::make_integer_sequence<int, NLAYERS>{});
+        run_iteration<CF>(Ls, R, Rl, t, P.beta, P.beta3, std::make_integer_sequence<int, NLAYERS>{});
         it += 1;
         conv = syndrome_pass<CF, 0>(Ls, t, active);
         done = (conv | slot_any<CF>(__ballot(it >= P.max_iter))) & active;
@@ -450,12 +483,13 @@ __global__ void __launch_bounds__(CF::THREADS) k_multi_wave(MultiParams P) {
 template <class CF>
 __global__ void __launch_bounds__(CF::THREADS) k_multi_group(MultiParams P) {
     __shared__ __attribute__((aligned(16))) uint16_t Ls[CF::N];
-    __shared__ uint32_t Rl[CF::NRL > 0 ? CF::NRL * CF::G : 1];
+    __shared__ uint32_t Rl_s[CF::NRL > 0 ? CF::NRL * CF::THREADS : 1];
     __shared__ uint32_t hist_s[HIST_LDS];
     __shared__ uint32_t flag_s[CF::WPW];
     __shared__ uint32_t cw_s, be_s;
     const Run run = load_run(P);
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    uint32_t *Rl = Rl_s + t;
     for (int i = t; i < HIST_LDS; i += CF::THREADS) hist_s[i] = 0u;
     const uint32_t n_wg = gridDim.x;
     const uint32_t n_static = (uint32_t)(((uint64_t)run.n_cw * 7u / 8u) / n_wg * n_wg);
@@ -486,7 +520,7 @@ __global__ void __launch_bounds__(CF::THREADS) k_multi_group(MultiParams P) {
 #pragma unroll
         for (int i = 0; i < CF::NR; ++i) R[i] = 0u;
 #pragma unroll
-        for (int i = 0; i < CF::NRL; ++i) Rl[i * CF::G + t] = 0u;
+        for (int i = 0; i < CF::NRL; ++i) Rl[i * CF::THREADS] = 0u;
         __syncthreads();
         int it = 0;
         bool ok = false;

@@ -236,7 +236,10 @@ int ldpc_autotune(const ldpc_code_desc *code, const ldpc_decoder_cfg *cfg, int d
  * variant, Eb/N0 and codeword stream, run as one captured HIP graph per
  * round: every code's persistent kernel takes codewords from its own device
  * queue; a round simulates up to pts[i].chunk_cw codewords of every code
- * that has not reached its max_blocks / target_fe.  Replaces running the
+ * that has not reached its max_blocks / target_fe.  Inside the graph the
+ * codes decoded one codeword per workgroup run in sequence and the codes with
+ * several codewords per workgroup run beside them, so kernels may overlap;
+ * results do not depend on it.  Replaces running the
  * reference's engine once per code (run_simulation.py:164-256 per matrix). */
 typedef struct ldpc_batch ldpc_batch;
 

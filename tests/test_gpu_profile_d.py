@@ -1,7 +1,8 @@
 """Profile D (the learned layered min-sum with a syndrome early exit,
 decode_multi.hip) against the oracle, bit for bit: final LLRs (bf16),
 iterations and convergence of channel dumps; Monte Carlo counters and
-histograms; the channel is the Profile T stream."""
+histograms; the channel is the Profile T stream.  K=16384 r1/2 has its own
+kernel (messages in an HBM workspace, L in all of the LDS)."""
 import numpy as np
 import pytest
 
@@ -11,7 +12,7 @@ from ldpc_mi355x import codes, decoder
 
 pytestmark = pytest.mark.gpu
 
-CASES = [(1024, "1/2", 1.6), (1024, "4/5", 3.0), (4096, "2/3", 2.0), (1024, "2/3", 2.4)]
+CASES = [(1024, "1/2", 1.6), (1024, "4/5", 3.0), (4096, "2/3", 2.0), (1024, "2/3", 2.4), (16384, "1/2", 0.9)]
 
 
 def params(code, G, seed):
@@ -25,7 +26,7 @@ def params(code, G, seed):
 def test_decode_matches_oracle(oracle, k, rate, ebn0, G, corr, r_max):
     code = codes.ar4ja(k, rate)
     mu, sg = channel_params(ebn0, code.k / code.n_tx)
-    n = {1024: 128, 4096: 24}[k]
+    n = {1024: 128, 4096: 24, 16384: 6}[k]
     llr = oracle.gen_channel(code, 5, 0, n, mu, sg)
     a, b = params(code, G, k + G)
     print(f"Profile D decode K={k} rate={rate} G={G} corr={corr} r_max={r_max}", flush=True)
@@ -52,6 +53,53 @@ def test_monte_carlo_matches_oracle(oracle, k, rate, ebn0):
     assert (got.blocks, got.frame_errors, got.bit_errors, got.iter_sum) == \
         (exp["blocks"], exp["frame_errors"], exp["bit_errors"], exp["iter_sum"])
     np.testing.assert_array_equal(got.hist, exp["hist"])
+
+
+def _params_k16384(code):
+    rng = np.random.default_rng(7)
+    p = DO.n_params(code)
+    return rng.uniform(0.8, 0.9, (1, p)).astype(np.float32), rng.uniform(0.0, 0.1, (1, p)).astype(np.float32)
+
+
+def test_monte_carlo_k16384_matches_oracle(oracle):
+    """40 codewords (all through the queue: the workgroup-wide broadcast of
+    the queued id) at 1.1 dB, converged and failed codewords."""
+    code = codes.ar4ja(16384, "1/2")
+    mu, sg = channel_params(1.1)
+    a, b = _params_k16384(code)
+    print("Profile D Monte Carlo K=16384 r1/2, 40 codewords", flush=True)
+    with decoder.Engine(16384, "1/2", max_iter=30, profile="D") as eng:
+        eng.set_dmin_params(a, b, True)
+        got = eng.run_point(1.1, 40, seed=23, first_cw=4, chunk_cw=40)
+    llr = oracle.gen_channel(code, 23, 4, 40, mu, sg)
+    exp = DO.run_point_d(code, llr, a, b, 30, corr=True)
+    assert exp["frame_errors"] > 0 and exp["iter_sum"] < 30 * 40
+    assert (got.blocks, got.frame_errors, got.bit_errors, got.iter_sum) == \
+        (exp["blocks"], exp["frame_errors"], exp["bit_errors"], exp["iter_sum"])
+    np.testing.assert_array_equal(got.hist, exp["hist"])
+
+
+def test_monte_carlo_k16384_static_split_agrees_with_decode():
+    """320 codewords over 256 workgroups: 256 from the static split, 64 from
+    the queue.  The Monte Carlo counters and histogram equal the tallies of
+    decoding the same channel (gen_channel + decode_llrs, whose arithmetic the
+    decode test pins to the oracle)."""
+    code = codes.ar4ja(16384, "1/2")
+    a, b = _params_k16384(code)
+    n, max_iter = 320, 30
+    print("Profile D K=16384 r1/2: static split, 320 codewords", flush=True)
+    with decoder.Engine(16384, "1/2", max_iter=max_iter, profile="D") as eng:
+        eng.set_dmin_params(a, b, True)
+        got = eng.run_point(1.1, n, seed=29, first_cw=0, chunk_cw=n)
+        llr = eng.gen_channel(29, 0, n, 1.1)
+        out, it, ok = eng.decode_llrs(llr)
+    fail = ok == 0
+    be = int(((out[fail, :code.k] & 0x8000) != 0).sum() + (out[fail, :code.k] == 0).sum())
+    hist = np.bincount(np.where(ok == 1, it, 0), minlength=max_iter + 1).astype(np.uint64)
+    assert (got.blocks, got.frame_errors, got.iter_sum) == (n, int(fail.sum()), int(it.sum()))
+    assert got.bit_errors == be
+    np.testing.assert_array_equal(got.hist, hist)
+    assert 0 < got.frame_errors < n
 
 
 def test_same_channel_as_profile_t():

@@ -919,6 +919,286 @@ __global__ void __launch_bounds__(CF::THREADS) k_d_group(MultiParams P) {
     hist_flush(hist_s, P, CF::THREADS);
 }
 
+#if LDPC_KID == 2 && LDPC_RID == 0
+// ---------------------------------------------------------------------------
+// Profile D for K = 16384 r1/2 (z = 2048).  One codeword per 1024-thread
+// workgroup, thread t updates checks t and t + 1024 of every layer.  The
+// codeword's fp32 L (40960 x 4 B = 160 KiB) takes all of the CU's LDS, and
+// its fp32 messages (2 x 60 per thread, 480 KiB per codeword) do not fit the
+// 128 VGPRs a 1024-thread workgroup gets, so they live in a per-workgroup
+// HBM workspace laid out [message][thread] (coalesced across the
+// workgroup).  A layer's messages are loaded one layer ahead (each message
+// belongs to one thread and one layer, so the prefetch never races a
+// store), and iteration 0 uses R = 0 without reading, so the workspace is
+// never cleared.  The workgroup-wide syndrome flag and the queue broadcast
+// go through control words at the end of the workspace with workgroup-scope
+// atomics (there is no LDS left for them); they are double-buffered by
+// iteration parity, so one barrier per iteration suffices.
+// ---------------------------------------------------------------------------
+constexpr ar4ja::Schedule SDB = ar4ja::make_schedule(2, 0, 0);
+constexpr int db_r_off(int li) {
+    int o = 0;
+    for (int i = 0; i < li; ++i) o += SDB.layer[i].deg;
+    return o;
+}
+
+struct CfgDB {
+    static constexpr int C = 2, ORD = 0;
+    static constexpr ar4ja::Schedule S = SDB;
+    static constexpr int Z = S.Z, N = S.N, NP = S.n_punct, NTX = S.N - S.n_punct, K = S.k_info;
+    static constexpr int THREADS = 1024, G = THREADS, WPW = THREADS / 64, CW_PER_WG = 1;
+    static constexpr bool WAVE = false;
+    static constexpr int r_off(int li) { return db_r_off(li); }
+    static constexpr int NR = db_r_off(NLAYERS);         // messages per check = parameters per row
+    static constexpr int CTL = 2 * NR * THREADS;         // control words: flag[2], queue broadcast
+    static constexpr int WORK_DWORDS = CTL + 64;         // per workgroup (control padded to 256 B)
+    static constexpr int DMAX = 6;
+    static_assert(Z == 2 * THREADS && N * 4 == 163840, "K = 16384 r1/2 layout");
+};
+
+// Workgroup barrier that orders LDS only: the messages in HBM are private to
+// their thread, so a layer need not wait for its message stores (or the next
+// layer's prefetch) to reach memory before the next layer starts.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// Control words are shared only by the waves of one workgroup (one CU), so
+// their atomics are workgroup-scoped: agent scope would write back and
+// invalidate the XCD's L2 on every release / acquire, and the L2 is full of
+// dirty message lines.  A control word reads the same in every lane; saying
+// so (readfirstlane) keeps the loop state that depends on it scalar.
+__device__ __forceinline__ uint32_t ctl_load(const uint32_t *p) {
+    return __builtin_amdgcn_readfirstlane(__hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP));
+}
+__device__ __forceinline__ void ctl_store(uint32_t *p, uint32_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// Messages of layer LI for the thread's two checks: element (d, h) at
+// Rbase[(2 (R0 + d) + h) * THREADS + t].
+template <int LI>
+__device__ __forceinline__ void load_r_db(const float *Rbase, int t, float (&buf)[2 * CfgDB::DMAX]) {
+    constexpr int D = CfgDB::S.layer[LI].deg, R0 = CfgDB::r_off(LI);
+    asm volatile("" : "+v"(t));  // addresses are formed here, per layer
+    const float *Rw = Rbase + t;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        buf[2 * d] = Rw[(2 * (R0 + d)) * CfgDB::THREADS];
+        buf[2 * d + 1] = Rw[(2 * (R0 + d) + 1) * CfgDB::THREADS];
+    }
+}
+
+// One check of layer LI (index j) with its messages in r[]: the arithmetic of
+// run_layer_d, operation for operation.
+template <int LI>
+__device__ __forceinline__ void check_db(float *Ls, int j, float *r, const float *alv, const float *bev, bool corr,
+                                         float r_max) {
+    constexpr ar4ja::Layer Ly = CfgDB::S.layer[LI];
+    constexpr int D = Ly.deg, R0 = CfgDB::r_off(LI), Z = CfgDB::Z;
+    asm volatile("" : "+v"(j));  // keeps the shifted LDS addresses inside the layer (not hoisted, no VGPRs pinned)
+    int v[D];
+    float q[D];
+    float m1 = __builtin_inff(), m2 = __builtin_inff();
+    int i1 = 0;
+    uint32_t s = 0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        v[d] = Ly.slot[d].base + ((j + Ly.slot[d].shift) & (Z - 1));
+        q[d] = __fsub_rn(Ls[v[d]], r[d]);
+        s ^= f2u(q[d]) >> 31;
+        const float a = fabsf(q[d]);
+        if (a < m1) {
+            m2 = m1;
+            m1 = a;
+            i1 = d;
+        } else if (a < m2) {
+            m2 = a;
+        }
+    }
+    float other = m1;
+    if (corr) {
+        const float C1 = 0.69140625f, K1 = -0.3984375f, C2 = 0.65625f, K2 = -0.326171875f;
+        const float x = __fsub_rn(m2, m1);
+        const float l1 = __fadd_rn(__fmul_rn(K1, x), C1);
+        const float l2 = __fadd_rn(__fmul_rn(K2, x), C2);
+        const float lm = l1 >= l2 ? l1 : l2;
+        const float o = __fsub_rn(m1, lm > 0.0f ? lm : 0.0f);
+        other = o > 0.0f ? o : 0.0f;
+    }
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        const float g = d == i1 ? m2 : other;
+        const float z = __fsub_rn(__fmul_rn(alv[d], g), bev[d]);
+        float mag = z > 0.0f ? z : 0.0f;
+        if (r_max > 0.0f && !(mag < r_max)) mag = r_max;
+        const uint32_t sb = (s ^ (f2u(q[d]) >> 31)) & 1u;
+        const float rr = u2f(f2u(mag) | (sb << 31));
+        Ls[v[d]] = __fadd_rn(q[d], rr);
+        r[d] = rr;
+    }
+}
+
+// Layer LI: messages arrive in cur (loaded one layer earlier; replaced by
+// zero in iteration 0); the next layer's are requested into nxt before the
+// compute.
+template <int LI>
+__device__ __forceinline__ void layer_db(float *Ls, float *Rbase, int t, float (&cur)[2 * CfgDB::DMAX],
+                                         float (&nxt)[2 * CfgDB::DMAX], bool first, const float *al, const float *be,
+                                         bool corr, float r_max) {
+    constexpr int D = CfgDB::S.layer[LI].deg, R0 = CfgDB::r_off(LI);
+    constexpr int LN = LI + 1 < NLAYERS ? LI + 1 : 0;
+    // this layer's parameters first, then the prefetch: vector-memory
+    // requests complete in order, so waiting for the parameters does not
+    // wait for the prefetch
+    float alv[CfgDB::DMAX], bev[CfgDB::DMAX];
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        alv[d] = al[R0 + d];
+        bev[d] = be[R0 + d];
+    }
+    asm volatile("" ::: "memory");
+    // unconditional (iteration 0 loads stale values that its next layer
+    // ignores): a branch here would make the wait for the parameters above
+    // also wait for the prefetch on the path that issued it
+    load_r_db<LN>(Rbase, t, nxt);
+    float ra[CfgDB::DMAX], rb[CfgDB::DMAX];
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        ra[d] = first ? 0.0f : cur[2 * d];
+        rb[d] = first ? 0.0f : cur[2 * d + 1];
+    }
+    check_db<LI>(Ls, t, ra, alv, bev, corr, r_max);
+    check_db<LI>(Ls, t + CfgDB::THREADS, rb, alv, bev, corr, r_max);
+    int tw = t;
+    asm volatile("" : "+v"(tw));
+    float *Rw = Rbase + tw;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        Rw[(2 * (R0 + d)) * CfgDB::THREADS] = ra[d];
+        Rw[(2 * (R0 + d) + 1) * CfgDB::THREADS] = rb[d];
+    }
+    lds_barrier();
+}
+
+template <int... LI>
+__device__ __forceinline__ void iteration_db(float *Ls, float *Rbase, int t, float (&b0)[2 * CfgDB::DMAX],
+                                             float (&b1)[2 * CfgDB::DMAX], bool first, const float *al,
+                                             const float *be, bool corr, float r_max,
+                                             std::integer_sequence<int, LI...>) {
+    // layer LI reads buffer LI & 1 and prefetches into the other; the last
+    // layer prefetches layer 0 of the next iteration
+    ((LI & 1 ? layer_db<LI>(Ls, Rbase, t, b1, b0, first, al, be, corr, r_max)
+             : layer_db<LI>(Ls, Rbase, t, b0, b1, first, al, be, corr, r_max)),
+     ...);
+}
+
+template <int LI>
+__device__ __forceinline__ bool fails_from_db(const float *Ls, int t) {
+    if constexpr (LI == NLAYERS) {
+        return false;
+    } else {
+        asm volatile("" ::: "memory");
+        const bool f = layer_fails_d<CfgDB, LI>(Ls, t) || layer_fails_d<CfgDB, LI>(Ls, t + CfgDB::THREADS);
+        if (__ballot(f) != 0) return true;
+        return fails_from_db<LI + 1>(Ls, t);
+    }
+}
+
+__global__ void __launch_bounds__(CfgDB::THREADS) k_d_big(MultiParams P) {
+    __shared__ __attribute__((aligned(16))) float Ls[CfgDB::N];  // all 160 KiB of LDS
+    const Run run = load_run(P);
+    const int t = threadIdx.x, lane = t & 63;
+    uint32_t *work = P.work + (size_t)blockIdx.x * CfgDB::WORK_DWORDS;
+    float *Rbase = reinterpret_cast<float *>(work);
+    uint32_t *ctl = work + CfgDB::CTL;  // [0], [1]: syndrome flags by iteration parity; [2]: queue broadcast
+    const uint32_t n_wg = gridDim.x;
+    const uint32_t n_static = (uint32_t)(((uint64_t)run.n_cw * 7u / 8u) / n_wg * n_wg);
+    const bool corr = P.d_corr != 0;
+    uint32_t kstat = 0;
+    unsigned long long blocks = 0, fe = 0, its = 0;
+    float b0[2 * CfgDB::DMAX], b1[2 * CfgDB::DMAX];
+#pragma unroll
+    for (int i = 0; i < 2 * CfgDB::DMAX; ++i) b0[i] = b1[i] = 0.0f;
+    for (;;) {
+        uint32_t cw;
+        const uint32_t s = blockIdx.x + kstat * n_wg;
+        if (s < n_static) {
+            cw = s;
+            ++kstat;
+        } else {
+            if (t == 0) {
+                const uint32_t v = atomicAdd(P.queue, 1u);
+                ctl_store(ctl + 2, v >= run.n_cw - n_static ? run.n_cw : n_static + v);
+            }
+            __syncthreads();
+            cw = ctl_load(ctl + 2);
+        }
+        if (cw >= run.n_cw) break;
+        if (P.llr_in)
+            copy_in_d<CfgDB>(Ls, P.llr_in + (size_t)cw * CfgDB::N, t);
+        else
+            gen_channel_d<CfgDB>(Ls, t, run.seed, run.first_cw + cw, run.mu, run.sigma);
+        if (t == 0) {
+            ctl_store(ctl + 0, 0u);
+            ctl_store(ctl + 1, 0u);
+        }
+        __syncthreads();
+        int it = 0;
+        bool ok = false;
+        while (it < P.max_iter) {
+            const int row = d_row(P, it);
+            iteration_db(Ls, Rbase, t, b0, b1, it == 0, P.d_alpha + row * CfgDB::NR, P.d_beta + row * CfgDB::NR, corr,
+                         P.d_rmax, std::make_integer_sequence<int, NLAYERS>{});
+            ++it;
+            // flag of this iteration; the other parity's was read an iteration
+            // (12 barriers) ago and is cleared for the next one
+            const bool f = fails_from_db<0>(Ls, t);
+            if (lane == 0 && f) __hip_atomic_fetch_or(ctl + (it & 1), 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (t == 0) ctl_store(ctl + ((it + 1) & 1), 0u);
+            __syncthreads();
+            if (ctl_load(ctl + (it & 1)) == 0u) {
+                ok = true;
+                break;
+            }
+        }
+        if (!ok) {
+            const unsigned long long e = wave_sum((unsigned long long)info_errors_d<CfgDB>(Ls, t));
+            if (lane == 0 && e) atomicAdd(P.counters + 2, e);
+        }
+        if (P.llr_in) copy_out_d<CfgDB>(P.llr_out + (size_t)cw * CfgDB::N, Ls, t);
+        if (t == 0) {
+            blocks += 1;
+            its += ok ? (unsigned)it : (unsigned)P.max_iter;
+            fe += ok ? 0u : 1u;
+            atomicAdd(P.hist + (ok ? it : 0), 1ull);
+            if (!ok && !P.llr_in) capture_failure(P.cap, run.first_cw + cw);
+            if (P.llr_in) {
+                P.iters[cw] = ok ? (uint32_t)it : (uint32_t)P.max_iter;
+                P.ok[cw] = ok ? 1 : 0;
+            }
+        }
+        __syncthreads();  // Ls is rewritten by the next codeword
+    }
+    if (t == 0) flush_counters(P, blocks, fe, 0ull, its);
+}
+
+hipError_t launch_db(const MultiParams &p, int grid, hipStream_t st) {
+    hipLaunchKernelGGL(k_d_big, dim3(grid), dim3(CfgDB::THREADS), 0, st, p);
+    return hipGetLastError();
+}
+
+hipError_t occupancy_db(int *wg_per_cu) {
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(wg_per_cu, k_d_big, CfgDB::THREADS, 0);
+}
+
+hipError_t attributes_db(hipFuncAttributes *a) {
+    return hipFuncGetAttributes(a, reinterpret_cast<const void *>(k_d_big));
+}
+#endif
+
 template <class CF>
 constexpr auto kernel_of_d() {
     if constexpr (CF::WAVE)
@@ -967,11 +1247,13 @@ const MultiVariant TABLE[] = {
 #endif
 };
 
-// Profile D (one check per thread: not for K = 16384 r1/2, z = 2048)
+// Profile D: one check per thread, except K = 16384 r1/2 (z = 2048), which
+// has its own kernel with the messages in an HBM workspace
 #if !(LDPC_KID == 2 && LDPC_RID == 0)
 const MultiVariant TABLE_D[] = {variant_d<CfgD<0>>()};
 #else
-const MultiVariant *const TABLE_D = nullptr;
+const MultiVariant TABLE_D[] = {MultiVariant{2, 0, CfgDB::C, CfgDB::ORD, CfgDB::THREADS, 1, &launch_db, &occupancy_db,
+                                             &attributes_db, 1, sizeof(uint32_t) * (size_t)CfgDB::WORK_DWORDS}};
 #endif
 static_assert(C1_OK == !(LDPC_KID == 2 && LDPC_RID == 0), "C = 1 availability");
 
@@ -986,12 +1268,6 @@ const MultiVariant *LDPC_TABLE_NAME(LDPC_KID, LDPC_RID)(int *n) {
     return TABLE;
 }
 
-const MultiVariant *LDPC_TABLE_D_NAME(LDPC_KID, LDPC_RID)() {
-#if !(LDPC_KID == 2 && LDPC_RID == 0)
-    return &TABLE_D[0];
-#else
-    return nullptr;
-#endif
-}
+const MultiVariant *LDPC_TABLE_D_NAME(LDPC_KID, LDPC_RID)() { return &TABLE_D[0]; }
 
 }  // namespace ldpc

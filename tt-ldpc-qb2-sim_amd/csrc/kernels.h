@@ -83,7 +83,8 @@ struct MultiParams {
     const float *d_alpha, *d_beta;
     int d_groups, d_corr;
     float d_rmax;
-    Capture cap;  // Monte Carlo mode only
+    Capture cap;    // Monte Carlo mode only
+    uint32_t *work;  // per-workgroup workspace (MultiVariant::work_per_wg bytes each), when the kernel needs one
 };
 struct MultiVariant {
     int kid, rid, c, order;  // code (K = 1024 << 2 kid, rate 1/2, 2/3, 4/5), checks per thread, layer order
@@ -92,6 +93,7 @@ struct MultiVariant {
     hipError_t (*occupancy)(int *wg_per_cu);
     hipError_t (*attributes)(hipFuncAttributes *a);
     int profile_d;  // 1: the learned min-sum kernel (Profile D), 0: Profile T
+    size_t work_per_wg = 0;  // bytes of HBM workspace per workgroup of the launch grid (0: none)
 };
 const MultiVariant *multi_table_k0_r0(int *n);
 const MultiVariant *multi_table_k0_r1(int *n);
@@ -102,7 +104,7 @@ const MultiVariant *multi_table_k1_r2(int *n);
 const MultiVariant *multi_table_k2_r0(int *n);
 const MultiVariant *multi_table_k2_r1(int *n);
 const MultiVariant *multi_table_k2_r2(int *n);
-// Profile D kernel of a code (null where not built: K = 16384 r1/2)
+// Profile D kernel of a code
 const MultiVariant *multi_table_d_k0_r0();
 const MultiVariant *multi_table_d_k0_r1();
 const MultiVariant *multi_table_d_k0_r2();

@@ -44,6 +44,7 @@ struct ldpc_ctx {
     // trapping-set capture (ldpc_set_capture)
     uint32_t *d_cap_count = nullptr;
     unsigned long long *d_cap_ids = nullptr;
+    void *d_work = nullptr;  // per-workgroup workspace of the Profile D kernel for z = 2048
     uint32_t cap_max = 0;
     // Profile D parameters on the device
     float *d_alpha = nullptr, *d_beta = nullptr;
@@ -322,8 +323,6 @@ int ldpc_create(const ldpc_code_desc *code, const ldpc_decoder_cfg *cfg, int dev
     if (prof_d) {
         if (code->family != LDPC_CODE_AR4JA)
             return fail(nullptr, LDPC_ERR_UNSUPPORTED, "Profile D kernels are built for the AR4JA codes");
-        if (!ldpc::profile_d_kernel(k_id, rate_id))
-            return fail(nullptr, LDPC_ERR_UNSUPPORTED, "Profile D needs z <= 1024 (not AR4JA K=16384 r1/2)");
         if (cfg->variant || cfg->l_max != 0.0f || cfg->r_max != 0.0f || cfg->l_post_max != 0.0f || cfg->gamma != 0.0f)
             return fail(nullptr, LDPC_ERR_ARG, "Profile D takes its settings from ldpc_set_dmin_params");
     } else if (prof_t) {
@@ -379,6 +378,9 @@ int ldpc_create(const ldpc_code_desc *code, const ldpc_decoder_cfg *cfg, int dev
         int occ = 0;
         if ((e = c->mv->occupancy(&occ)) != hipSuccess) return bail(e, "occupancy");
         c->grid_mc = c->grid_dec = cus * (occ > 0 ? occ : 1);
+        if (c->mv->work_per_wg &&
+            (e = hipMalloc(&c->d_work, (size_t)c->grid_mc * c->mv->work_per_wg)) != hipSuccess)
+            return bail(e, "hipMalloc");
     } else if (prof_t && !(is_r12_k1024(*code) && cfg->variant == 0)) {
         c->mv = ldpc::find_variant(k_id, rate_id, cfg->variant ? cfg->variant : ldpc::default_variant(k_id, rate_id));
         ldpc::fill_code_info(k_id, rate_id, c->info);
@@ -440,6 +442,7 @@ void ldpc_destroy(ldpc_ctx *c) {
     if (c->d_alpha) (void)hipFree(c->d_alpha);
     if (c->d_cap_count) (void)hipFree(c->d_cap_count);
     if (c->d_cap_ids) (void)hipFree(c->d_cap_ids);
+    if (c->d_work) (void)hipFree(c->d_work);
     if (c->d_beta) (void)hipFree(c->d_beta);
     if (c->d_queue) (void)hipFree(c->d_queue);
     if (c->d_counters) (void)hipFree(c->d_counters);
@@ -579,6 +582,7 @@ static ldpc::MultiParams multi_params(const ldpc_ctx *c) {
     p.d_corr = c->d_corr;
     p.d_rmax = c->d_rmax;
     p.cap = ldpc::Capture{c->d_cap_count, c->d_cap_ids, c->cap_max};
+    p.work = static_cast<uint32_t *>(c->d_work);
     p.queue = c->d_queue;
     p.counters = c->d_counters;
     p.hist = c->d_hist;

@@ -926,8 +926,13 @@ __global__ void __launch_bounds__(CF::THREADS) k_d_group(MultiParams P) {
 // codeword's fp32 L (40960 x 4 B = 160 KiB) takes all of the CU's LDS, and
 // its fp32 messages (2 x 60 per thread, 480 KiB per codeword) do not fit the
 // 128 VGPRs a 1024-thread workgroup gets, so they live in a per-workgroup
-// HBM workspace laid out [message][thread] (coalesced across the
-// workgroup).  A layer's messages are loaded one layer ahead (each message
+// HBM workspace laid out [word][thread] (coalesced across the workgroup).
+// The kernel is bound by that traffic, so a check of degree above 3 stores
+// its messages in three words, from which they are rebuilt bit for bit
+// with the previous iteration's parameter row: g at the first minimum
+// (min2), g elsewhere (min1, or its corrected value), and the first
+// minimum's index with the message signs (292 KiB per codeword-iteration
+// instead of 480).  A layer's words are loaded one layer ahead (each word
 // belongs to one thread and one layer, so the prefetch never races a
 // store), and iteration 0 uses R = 0 without reading, so the workspace is
 // never cleared.  The workgroup-wide syndrome flag and the queue broadcast
@@ -941,6 +946,14 @@ constexpr int db_r_off(int li) {
     for (int i = 0; i < li; ++i) o += SDB.layer[i].deg;
     return o;
 }
+// workspace words per check of layer li: the raw messages up to degree 3,
+// else the compressed form
+constexpr int db_words(int li) { return SDB.layer[li].deg <= 3 ? SDB.layer[li].deg : 3; }
+constexpr int db_w_off(int li) {
+    int o = 0;
+    for (int i = 0; i < li; ++i) o += db_words(i);
+    return o;
+}
 
 struct CfgDB {
     static constexpr int C = 2, ORD = 0;
@@ -949,10 +962,12 @@ struct CfgDB {
     static constexpr int THREADS = 1024, G = THREADS, WPW = THREADS / 64, CW_PER_WG = 1;
     static constexpr bool WAVE = false;
     static constexpr int r_off(int li) { return db_r_off(li); }
+    static constexpr int w_off(int li) { return db_w_off(li); }
     static constexpr int NR = db_r_off(NLAYERS);         // messages per check = parameters per row
-    static constexpr int CTL = 2 * NR * THREADS;         // control words: flag[2], queue broadcast
+    static constexpr int NW = db_w_off(NLAYERS);         // workspace words per check
+    static constexpr int CTL = 2 * NW * THREADS;         // control words: flag[2], queue broadcast
     static constexpr int WORK_DWORDS = CTL + 64;         // per workgroup (control padded to 256 B)
-    static constexpr int DMAX = 6;
+    static constexpr int DMAX = 6, WMAX = 3;
     static_assert(Z == 2 * THREADS && N * 4 == 163840, "K = 16384 r1/2 layout");
 };
 
@@ -977,46 +992,72 @@ __device__ __forceinline__ void ctl_store(uint32_t *p, uint32_t v) {
     __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
-// Messages of layer LI for the thread's two checks: element (d, h) at
-// Rbase[(2 (R0 + d) + h) * THREADS + t].
+// Workspace words of layer LI for the thread's two checks: word (k, h) at
+// Rbase[(2 (W0 + k) + h) * THREADS + t].
 template <int LI>
-__device__ __forceinline__ void load_r_db(const float *Rbase, int t, float (&buf)[2 * CfgDB::DMAX]) {
-    constexpr int D = CfgDB::S.layer[LI].deg, R0 = CfgDB::r_off(LI);
+__device__ __forceinline__ void load_r_db(const uint32_t *Rbase, int t, uint32_t (&buf)[2 * CfgDB::WMAX]) {
+    constexpr int W = db_words(LI), W0 = CfgDB::w_off(LI);
     asm volatile("" : "+v"(t));  // addresses are formed here, per layer
-    const float *Rw = Rbase + t;
+    const uint32_t *Rw = Rbase + t;
 #pragma unroll
-    for (int d = 0; d < D; ++d) {
-        buf[2 * d] = Rw[(2 * (R0 + d)) * CfgDB::THREADS];
-        buf[2 * d + 1] = Rw[(2 * (R0 + d) + 1) * CfgDB::THREADS];
+    for (int k = 0; k < W; ++k) {
+        buf[2 * k] = Rw[(2 * (W0 + k)) * CfgDB::THREADS];
+        buf[2 * k + 1] = Rw[(2 * (W0 + k) + 1) * CfgDB::THREADS];
     }
 }
 
-// One check of layer LI (index j) with its messages in r[]: the arithmetic of
-// run_layer_d, operation for operation.
-template <int LI>
-__device__ __forceinline__ void check_db(float *Ls, int j, float *r, const float *alv, const float *bev, bool corr,
-                                         float r_max) {
+// Message magnitude from g and the edge's parameters (run_layer_d's
+// arithmetic); the same function rebuilds a stored message.  With a cap
+// (CAP: r_max > 0), min(mag, r_max) is run_layer_d's "r_max unless
+// mag < r_max" for the non-negative finite magnitudes here.
+template <bool CAP>
+__device__ __forceinline__ float db_mag(float a, float b, float g, float r_max) {
+    const float z = __fsub_rn(__fmul_rn(a, g), b);
+    const float mag = z > 0.0f ? z : 0.0f;
+    return CAP ? fminf(mag, r_max) : mag;
+}
+
+// One check of layer LI (index j): the arithmetic of run_layer_d, operation
+// for operation.  w[h + 2 k] holds the check's workspace words from the
+// previous iteration (ignored when first) and receives this iteration's.
+// For a compressed layer, pa / pb are the previous iteration's parameters.
+template <int LI, int H, bool CAP>
+__device__ __forceinline__ void check_db(float *Ls, int j, uint32_t *w, bool first, const float *pa,
+                                         const float *pb, const float *alv, const float *bev, bool corr,
+                                         float rcap) {
     constexpr ar4ja::Layer Ly = CfgDB::S.layer[LI];
-    constexpr int D = Ly.deg, R0 = CfgDB::r_off(LI), Z = CfgDB::Z;
+    constexpr int D = Ly.deg, Z = CfgDB::Z;
+    constexpr bool PACK = D > 3;
     asm volatile("" : "+v"(j));  // keeps the shifted LDS addresses inside the layer (not hoisted, no VGPRs pinned)
+    float r[D];
+    if constexpr (PACK) {
+        const float g1 = u2f(w[H]), go = u2f(w[H + 2]);
+        const uint32_t pk = w[H + 4];
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            const float mag = db_mag<CAP>(pa[d], pb[d], (pk & 7u) == (uint32_t)d ? g1 : go, rcap);
+            r[d] = first ? 0.0f : u2f(f2u(mag) | (((pk >> (3 + d)) & 1u) << 31));
+        }
+    } else {
+#pragma unroll
+        for (int d = 0; d < D; ++d) r[d] = first ? 0.0f : u2f(w[H + 2 * d]);
+    }
     int v[D];
     float q[D];
     float m1 = __builtin_inff(), m2 = __builtin_inff();
     int i1 = 0;
-    uint32_t s = 0;
+    uint32_t s = 0;  // bit 31: product of the signs
 #pragma unroll
     for (int d = 0; d < D; ++d) {
         v[d] = Ly.slot[d].base + ((j + Ly.slot[d].shift) & (Z - 1));
         q[d] = __fsub_rn(Ls[v[d]], r[d]);
-        s ^= f2u(q[d]) >> 31;
+        s ^= f2u(q[d]);
+        // "if a < m1: (m2, m1, i1) = (m1, a, d) elif a < m2: m2 = a" without
+        // branches: with m1 <= m2 the new m2 is the median of (a, m1, m2)
         const float a = fabsf(q[d]);
-        if (a < m1) {
-            m2 = m1;
-            m1 = a;
-            i1 = d;
-        } else if (a < m2) {
-            m2 = a;
-        }
+        i1 = a < m1 ? d : i1;
+        m2 = __builtin_amdgcn_fmed3f(a, m1, m2);
+        m1 = fminf(a, m1);
     }
     float other = m1;
     if (corr) {
@@ -1028,70 +1069,83 @@ __device__ __forceinline__ void check_db(float *Ls, int j, float *r, const float
         const float o = __fsub_rn(m1, lm > 0.0f ? lm : 0.0f);
         other = o > 0.0f ? o : 0.0f;
     }
+    uint32_t sbits = 0;
 #pragma unroll
     for (int d = 0; d < D; ++d) {
-        const float g = d == i1 ? m2 : other;
-        const float z = __fsub_rn(__fmul_rn(alv[d], g), bev[d]);
-        float mag = z > 0.0f ? z : 0.0f;
-        if (r_max > 0.0f && !(mag < r_max)) mag = r_max;
-        const uint32_t sb = (s ^ (f2u(q[d]) >> 31)) & 1u;
-        const float rr = u2f(f2u(mag) | (sb << 31));
+        const float mag = db_mag<CAP>(alv[d], bev[d], d == i1 ? m2 : other, rcap);
+        const uint32_t sw = s ^ f2u(q[d]);  // bit 31: the message's sign
+        const float rr = u2f(f2u(mag) | (sw & 0x80000000u));
         Ls[v[d]] = __fadd_rn(q[d], rr);
-        r[d] = rr;
+        if constexpr (PACK)
+            sbits |= (sw >> 31) << (3 + d);
+        else
+            w[H + 2 * d] = f2u(rr);
+    }
+    if constexpr (PACK) {
+        w[H] = f2u(m2);
+        w[H + 2] = f2u(other);
+        w[H + 4] = sbits | (uint32_t)i1;
     }
 }
 
-// Layer LI: messages arrive in cur (loaded one layer earlier; replaced by
-// zero in iteration 0); the next layer's are requested into nxt before the
-// compute.
-template <int LI>
-__device__ __forceinline__ void layer_db(float *Ls, float *Rbase, int t, float (&cur)[2 * CfgDB::DMAX],
-                                         float (&nxt)[2 * CfgDB::DMAX], bool first, const float *al, const float *be,
-                                         bool corr, float r_max) {
-    constexpr int D = CfgDB::S.layer[LI].deg, R0 = CfgDB::r_off(LI);
+// Layer LI: workspace words arrive in cur (loaded one layer earlier;
+// ignored in iteration 0); the next layer's are requested into nxt before
+// the compute.  al / be are this iteration's parameter row, pal / pbe the
+// previous iteration's.
+template <int LI, bool CAP>
+__device__ __forceinline__ void layer_db(float *Ls, uint32_t *Rbase, int t, uint32_t (&cur)[2 * CfgDB::WMAX],
+                                         uint32_t (&nxt)[2 * CfgDB::WMAX], bool first, const float *al,
+                                         const float *be, const float *pal, const float *pbe, bool corr,
+                                         float rcap) {
+    constexpr int D = CfgDB::S.layer[LI].deg, R0 = CfgDB::r_off(LI), W = db_words(LI), W0 = CfgDB::w_off(LI);
+    constexpr bool PACK = D > 3;
     constexpr int LN = LI + 1 < NLAYERS ? LI + 1 : 0;
-    // this layer's parameters first, then the prefetch: vector-memory
-    // requests complete in order, so waiting for the parameters does not
-    // wait for the prefetch
-    float alv[CfgDB::DMAX], bev[CfgDB::DMAX];
+    // The parameter rows are read through the constant address space, i.e.
+    // with scalar loads: a vector load here would wait (the vector-memory
+    // counter covers stores too, in order) for the previous layer's message
+    // stores to be acknowledged, which serialises every layer on a write
+    // round trip.
+    typedef const __attribute__((address_space(4))) float cfloat;
+    const cfloat *alc = (const cfloat *)al, *bec = (const cfloat *)be;
+    const cfloat *palc = (const cfloat *)pal, *pbec = (const cfloat *)pbe;
+    float alv[D], bev[D], pa[PACK ? D : 1], pb[PACK ? D : 1];
 #pragma unroll
     for (int d = 0; d < D; ++d) {
-        alv[d] = al[R0 + d];
-        bev[d] = be[R0 + d];
+        alv[d] = alc[R0 + d];
+        bev[d] = bec[R0 + d];
     }
-    asm volatile("" ::: "memory");
-    // unconditional (iteration 0 loads stale values that its next layer
-    // ignores): a branch here would make the wait for the parameters above
-    // also wait for the prefetch on the path that issued it
+    if constexpr (PACK) {
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            pa[d] = palc[R0 + d];
+            pb[d] = pbec[R0 + d];
+        }
+    }
+    // unconditional (iteration 0 loads stale words that its next layer
+    // ignores, and a branch would only add waits)
     load_r_db<LN>(Rbase, t, nxt);
-    float ra[CfgDB::DMAX], rb[CfgDB::DMAX];
-#pragma unroll
-    for (int d = 0; d < D; ++d) {
-        ra[d] = first ? 0.0f : cur[2 * d];
-        rb[d] = first ? 0.0f : cur[2 * d + 1];
-    }
-    check_db<LI>(Ls, t, ra, alv, bev, corr, r_max);
-    check_db<LI>(Ls, t + CfgDB::THREADS, rb, alv, bev, corr, r_max);
+    check_db<LI, 0, CAP>(Ls, t, cur, first, pa, pb, alv, bev, corr, rcap);
+    check_db<LI, 1, CAP>(Ls, t + CfgDB::THREADS, cur, first, pa, pb, alv, bev, corr, rcap);
     int tw = t;
     asm volatile("" : "+v"(tw));
-    float *Rw = Rbase + tw;
+    uint32_t *Rw = Rbase + tw;
 #pragma unroll
-    for (int d = 0; d < D; ++d) {
-        Rw[(2 * (R0 + d)) * CfgDB::THREADS] = ra[d];
-        Rw[(2 * (R0 + d) + 1) * CfgDB::THREADS] = rb[d];
+    for (int k = 0; k < W; ++k) {
+        Rw[(2 * (W0 + k)) * CfgDB::THREADS] = cur[2 * k];
+        Rw[(2 * (W0 + k) + 1) * CfgDB::THREADS] = cur[2 * k + 1];
     }
     lds_barrier();
 }
 
-template <int... LI>
-__device__ __forceinline__ void iteration_db(float *Ls, float *Rbase, int t, float (&b0)[2 * CfgDB::DMAX],
-                                             float (&b1)[2 * CfgDB::DMAX], bool first, const float *al,
-                                             const float *be, bool corr, float r_max,
-                                             std::integer_sequence<int, LI...>) {
+template <bool CAP, int... LI>
+__device__ __forceinline__ void iteration_db(float *Ls, uint32_t *Rbase, int t, uint32_t (&b0)[2 * CfgDB::WMAX],
+                                             uint32_t (&b1)[2 * CfgDB::WMAX], bool first, const float *al,
+                                             const float *be, const float *pal, const float *pbe, bool corr,
+                                             float rcap, std::integer_sequence<int, LI...>) {
     // layer LI reads buffer LI & 1 and prefetches into the other; the last
     // layer prefetches layer 0 of the next iteration
-    ((LI & 1 ? layer_db<LI>(Ls, Rbase, t, b1, b0, first, al, be, corr, r_max)
-             : layer_db<LI>(Ls, Rbase, t, b0, b1, first, al, be, corr, r_max)),
+    ((LI & 1 ? layer_db<LI, CAP>(Ls, Rbase, t, b1, b0, first, al, be, pal, pbe, corr, rcap)
+             : layer_db<LI, CAP>(Ls, Rbase, t, b0, b1, first, al, be, pal, pbe, corr, rcap)),
      ...);
 }
 
@@ -1107,21 +1161,24 @@ __device__ __forceinline__ bool fails_from_db(const float *Ls, int t) {
     }
 }
 
+// CAP: r_max > 0; CORR: the min* correction of g (both fixed per launch)
+template <bool CAP, bool CORR>
 __global__ void __launch_bounds__(CfgDB::THREADS) k_d_big(MultiParams P) {
     __shared__ __attribute__((aligned(16))) float Ls[CfgDB::N];  // all 160 KiB of LDS
     const Run run = load_run(P);
     const int t = threadIdx.x, lane = t & 63;
     uint32_t *work = P.work + (size_t)blockIdx.x * CfgDB::WORK_DWORDS;
-    float *Rbase = reinterpret_cast<float *>(work);
+    uint32_t *Rbase = work;
     uint32_t *ctl = work + CfgDB::CTL;  // [0], [1]: syndrome flags by iteration parity; [2]: queue broadcast
     const uint32_t n_wg = gridDim.x;
     const uint32_t n_static = (uint32_t)(((uint64_t)run.n_cw * 7u / 8u) / n_wg * n_wg);
-    const bool corr = P.d_corr != 0;
+    constexpr bool corr = CORR;
+    const float rcap = P.d_rmax;
     uint32_t kstat = 0;
     unsigned long long blocks = 0, fe = 0, its = 0;
-    float b0[2 * CfgDB::DMAX], b1[2 * CfgDB::DMAX];
+    uint32_t b0[2 * CfgDB::WMAX], b1[2 * CfgDB::WMAX];
 #pragma unroll
-    for (int i = 0; i < 2 * CfgDB::DMAX; ++i) b0[i] = b1[i] = 0.0f;
+    for (int i = 0; i < 2 * CfgDB::WMAX; ++i) b0[i] = b1[i] = 0u;
     for (;;) {
         uint32_t cw;
         const uint32_t s = blockIdx.x + kstat * n_wg;
@@ -1149,9 +1206,11 @@ __global__ void __launch_bounds__(CfgDB::THREADS) k_d_big(MultiParams P) {
         int it = 0;
         bool ok = false;
         while (it < P.max_iter) {
-            const int row = d_row(P, it);
-            iteration_db(Ls, Rbase, t, b0, b1, it == 0, P.d_alpha + row * CfgDB::NR, P.d_beta + row * CfgDB::NR, corr,
-                         P.d_rmax, std::make_integer_sequence<int, NLAYERS>{});
+            // the previous iteration's row rebuilds the stored messages
+            const int row = d_row(P, it), prow = d_row(P, it > 0 ? it - 1 : 0);
+            iteration_db<CAP>(Ls, Rbase, t, b0, b1, it == 0, P.d_alpha + row * CfgDB::NR, P.d_beta + row * CfgDB::NR,
+                         P.d_alpha + prow * CfgDB::NR, P.d_beta + prow * CfgDB::NR, corr, rcap,
+                         std::make_integer_sequence<int, NLAYERS>{});
             ++it;
             // flag of this iteration; the other parity's was read an iteration
             // (12 barriers) ago and is cleared for the next one
@@ -1186,16 +1245,20 @@ __global__ void __launch_bounds__(CfgDB::THREADS) k_d_big(MultiParams P) {
 }
 
 hipError_t launch_db(const MultiParams &p, int grid, hipStream_t st) {
-    hipLaunchKernelGGL(k_d_big, dim3(grid), dim3(CfgDB::THREADS), 0, st, p);
+    const bool cap = p.d_rmax > 0.0f, corr = p.d_corr != 0;
+    auto k = cap ? (corr ? &k_d_big<true, true> : &k_d_big<true, false>)
+                 : (corr ? &k_d_big<false, true> : &k_d_big<false, false>);
+    hipLaunchKernelGGL(k, dim3(grid), dim3(CfgDB::THREADS), 0, st, p);
     return hipGetLastError();
 }
 
+// the four instances share the launch shape and the LDS that bounds residency
 hipError_t occupancy_db(int *wg_per_cu) {
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(wg_per_cu, k_d_big, CfgDB::THREADS, 0);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(wg_per_cu, k_d_big<true, true>, CfgDB::THREADS, 0);
 }
 
 hipError_t attributes_db(hipFuncAttributes *a) {
-    return hipFuncGetAttributes(a, reinterpret_cast<const void *>(k_d_big));
+    return hipFuncGetAttributes(a, reinterpret_cast<const void *>(k_d_big<true, true>));
 }
 #endif
 

@@ -621,32 +621,32 @@ struct CfgD {
 
 template <class CF, int LI>
 __device__ __forceinline__ void run_layer_d(float *Ls, float (&R)[CF::NR], int t, const float *al, const float *be,
-                                            bool corr, float r_max) {
+                                            bool corr, float rcap) {
     constexpr ar4ja::Layer Ly = CF::S.layer[LI];
     constexpr int D = Ly.deg, R0 = CF::r_off(LI), Z = CF::Z;
     asm volatile("" : "+v"(t));
+    // parameter rows through the constant address space: scalar loads where
+    // the row is uniform (one codeword per workgroup), no VGPRs held for them
+    typedef const __attribute__((address_space(4))) float cfloat;
+    const cfloat *alc = (const cfloat *)al, *bec = (const cfloat *)be;
     int v[D];
     float q[D];
     float m1 = __builtin_inff(), m2 = __builtin_inff();
-    int i1 = 0, i2 = 0;
-    uint32_t s = 0;
+    int i1 = 0;
+    uint32_t s = 0;  // bit 31: product of the signs
 #pragma unroll
     for (int d = 0; d < D; ++d) {
         v[d] = Ly.slot[d].base + ((t + Ly.slot[d].shift) & (Z - 1));
         q[d] = __fsub_rn(Ls[v[d]], R[R0 + d]);
-        s ^= f2u(q[d]) >> 31;
+        s ^= f2u(q[d]);
+        // (m1, i1) the first minimum of |q| (first index on ties), m2 the
+        // second: "if a < m1: (m2, m1, i1) = (m1, a, d) elif a < m2: m2 = a"
+        // without branches; with m1 <= m2 the new m2 is med3(a, m1, m2)
         const float a = fabsf(q[d]);
-        if (a < m1) {
-            m2 = m1;
-            i2 = i1;
-            m1 = a;
-            i1 = d;
-        } else if (a < m2) {
-            m2 = a;
-            i2 = d;
-        }
+        i1 = a < m1 ? d : i1;
+        m2 = __builtin_amdgcn_fmed3f(a, m1, m2);
+        m1 = fminf(a, m1);
     }
-    (void)i2;
     float other = m1;
     if (corr) {
         const float C1 = 0.69140625f, K1 = -0.3984375f, C2 = 0.65625f, K2 = -0.326171875f;
@@ -660,11 +660,11 @@ __device__ __forceinline__ void run_layer_d(float *Ls, float (&R)[CF::NR], int t
 #pragma unroll
     for (int d = 0; d < D; ++d) {
         const float g = d == i1 ? m2 : other;
-        const float z = __fsub_rn(__fmul_rn(al[R0 + d], g), be[R0 + d]);
-        float mag = z > 0.0f ? z : 0.0f;
-        if (r_max > 0.0f && !(mag < r_max)) mag = r_max;
-        const uint32_t sb = (s ^ (f2u(q[d]) >> 31)) & 1u;
-        const float r = u2f(f2u(mag) | (sb << 31));
+        const float z = __fsub_rn(__fmul_rn(alc[R0 + d], g), bec[R0 + d]);
+        // rcap: r_max, or +inf without a cap; for these non-negative finite
+        // magnitudes min(mag, r_max) is "r_max unless mag < r_max"
+        const float mag = fminf(z > 0.0f ? z : 0.0f, rcap);
+        const float r = u2f(f2u(mag) | ((s ^ f2u(q[d])) & 0x80000000u));
         Ls[v[d]] = __fadd_rn(q[d], r);
         R[R0 + d] = r;
     }
@@ -672,11 +672,11 @@ __device__ __forceinline__ void run_layer_d(float *Ls, float (&R)[CF::NR], int t
 
 template <class CF, int... LI>
 __device__ __forceinline__ void run_iteration_d(float *Ls, float (&R)[CF::NR], int t, const float *al, const float *be,
-                                                bool corr, float r_max, std::integer_sequence<int, LI...>) {
+                                                bool corr, float rcap, std::integer_sequence<int, LI...>) {
     if constexpr (CF::WAVE)
-        ((run_layer_d<CF, LI>(Ls, R, t, al, be, corr, r_max), wave_fence()), ...);
+        ((run_layer_d<CF, LI>(Ls, R, t, al, be, corr, rcap), wave_fence()), ...);
     else
-        ((run_layer_d<CF, LI>(Ls, R, t, al, be, corr, r_max), __syncthreads()), ...);
+        ((run_layer_d<CF, LI>(Ls, R, t, al, be, corr, rcap), __syncthreads()), ...);
 }
 
 template <class CF, int LI>
@@ -765,6 +765,7 @@ __global__ void __launch_bounds__(CF::THREADS) k_d_wave(MultiParams P) {
     const uint32_t n_static = (uint32_t)(((uint64_t)run.n_cw * 7u / 8u) / n_slots * n_slots);
     const uint32_t all_slots = (1u << CF::SLOTS) - 1u;
     const bool corr = P.d_corr != 0;
+    const float rcap = P.d_rmax > 0.0f ? P.d_rmax : __builtin_inff();
     uint32_t kstat = 0, cw = run.n_cw;
     int it = 0;
     float R[CF::NR];
@@ -820,7 +821,7 @@ __global__ void __launch_bounds__(CF::THREADS) k_d_wave(MultiParams P) {
         }
         if (!active) break;
         const int row = d_row(P, it);
-        run_iteration_d<CF>(Ls, R, t, P.d_alpha + row * CF::NR, P.d_beta + row * CF::NR, corr, P.d_rmax,
+        run_iteration_d<CF>(Ls, R, t, P.d_alpha + row * CF::NR, P.d_beta + row * CF::NR, corr, rcap,
                             std::make_integer_sequence<int, NLAYERS>{});
         it += 1;
         conv = syndrome_pass_d<CF, 0>(Ls, t, active);
@@ -847,6 +848,7 @@ __global__ void __launch_bounds__(CF::THREADS) k_d_group(MultiParams P) {
     const uint32_t n_wg = gridDim.x;
     const uint32_t n_static = (uint32_t)(((uint64_t)run.n_cw * 7u / 8u) / n_wg * n_wg);
     const bool corr = P.d_corr != 0;
+    const float rcap = P.d_rmax > 0.0f ? P.d_rmax : __builtin_inff();
     uint32_t kstat = 0;
     unsigned long long blocks = 0, fe = 0, be = 0, its = 0;
     float R[CF::NR];
@@ -878,7 +880,7 @@ __global__ void __launch_bounds__(CF::THREADS) k_d_group(MultiParams P) {
         bool ok = false;
         while (it < P.max_iter) {
             const int row = d_row(P, it);
-            run_iteration_d<CF>(Ls, R, t, P.d_alpha + row * CF::NR, P.d_beta + row * CF::NR, corr, P.d_rmax,
+            run_iteration_d<CF>(Ls, R, t, P.d_alpha + row * CF::NR, P.d_beta + row * CF::NR, corr, rcap,
                                 std::make_integer_sequence<int, NLAYERS>{});
             ++it;
             const bool f = wave_fails_from_d<CF, 0>(Ls, t);

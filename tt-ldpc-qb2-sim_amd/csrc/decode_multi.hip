@@ -927,17 +927,18 @@ __global__ void __launch_bounds__(CF::THREADS) k_d_group(MultiParams P) {
 // workgroup, thread t updates checks t and t + 1024 of every layer.  The
 // codeword's fp32 L (40960 x 4 B = 160 KiB) takes all of the CU's LDS, and
 // its fp32 messages (2 x 60 per thread, 480 KiB per codeword) do not fit the
-// 128 VGPRs a 1024-thread workgroup gets, so they live in a per-workgroup
-// HBM workspace laid out [word][thread] (coalesced across the workgroup).
-// The kernel is bound by that traffic, so a check of degree above 3 stores
-// its messages in three words, from which they are rebuilt bit for bit
+// 128 VGPRs a 1024-thread workgroup gets.  A check of degree above 3 keeps
+// its messages as three words, from which they are rebuilt bit for bit
 // with the previous iteration's parameter row: g at the first minimum
 // (min2), g elsewhere (min1, or its corrected value), and the first
-// minimum's index with the message signs (292 KiB per codeword-iteration
-// instead of 480).  A layer's words are loaded one layer ahead (each word
-// belongs to one thread and one layer, so the prefetch never races a
-// store), and iteration 0 uses R = 0 without reading, so the workspace is
-// never cleared.  The workgroup-wide syndrome flag and the queue broadcast
+// minimum's index with the message signs; a degree-3 check keeps its three
+// messages.  That is six words per thread and layer: six layers' words stay
+// in registers, the other six go through a per-workgroup HBM workspace laid
+// out [word][thread] (coalesced across the workgroup), 144 KiB read and
+// written per codeword-iteration.  A workspace layer's words are loaded one
+// layer ahead (each word belongs to one thread and one layer, so the
+// prefetch never races a store), and iteration 0 uses R = 0 without
+// reading, so neither the registers nor the workspace are cleared.  The workgroup-wide syndrome flag and the queue broadcast
 // go through control words at the end of the workspace with workgroup-scope
 // atomics (there is no LDS left for them); they are double-buffered by
 // iteration parity, so one barrier per iteration suffices.
@@ -951,11 +952,30 @@ constexpr int db_r_off(int li) {
 // workspace words per check of layer li: the raw messages up to degree 3,
 // else the compressed form
 constexpr int db_words(int li) { return SDB.layer[li].deg <= 3 ? SDB.layer[li].deg : 3; }
+// Layers whose words go through the workspace; the others keep theirs in
+// registers, six per thread and layer (the 128 VGPRs a 1024-thread
+// workgroup allows hold six such layers without spilling in the loop;
+// seven spill 18 dwords per iteration).  No two workspace layers are
+// adjacent, cyclically, so one prefetch buffer serves them all, filled by
+// the layer before each.
+constexpr uint32_t DB_HBM = 0xAAAu;  // layers 1, 3, 5, 7, 9, 11
+constexpr bool db_hbm(int li) { return (DB_HBM >> li) & 1u; }
 constexpr int db_w_off(int li) {
     int o = 0;
-    for (int i = 0; i < li; ++i) o += db_words(i);
+    for (int i = 0; i < li; ++i) o += db_hbm(i) ? db_words(i) : 0;
     return o;
 }
+constexpr int db_reg_off(int li) {
+    int o = 0;
+    for (int i = 0; i < li; ++i) o += db_hbm(i) ? 0 : 2 * db_words(i);
+    return o;
+}
+constexpr bool db_hbm_apart() {
+    for (int i = 0; i < NLAYERS; ++i)
+        if (db_hbm(i) && db_hbm((i + 1) % NLAYERS)) return false;
+    return true;
+}
+static_assert(db_hbm_apart(), "workspace layers must not be adjacent");
 
 struct CfgDB {
     static constexpr int C = 2, ORD = 0;
@@ -967,6 +987,7 @@ struct CfgDB {
     static constexpr int w_off(int li) { return db_w_off(li); }
     static constexpr int NR = db_r_off(NLAYERS);         // messages per check = parameters per row
     static constexpr int NW = db_w_off(NLAYERS);         // workspace words per check
+    static constexpr int NREGW = db_reg_off(NLAYERS);    // register-resident words per thread
     static constexpr int CTL = 2 * NW * THREADS;         // control words: flag[2], queue broadcast
     static constexpr int WORK_DWORDS = CTL + 64;         // per workgroup (control padded to 256 B)
     static constexpr int DMAX = 6, WMAX = 3;
@@ -1091,16 +1112,17 @@ __device__ __forceinline__ void check_db(float *Ls, int j, uint32_t *w, bool fir
 }
 
 // Layer LI: workspace words arrive in cur (loaded one layer earlier;
-// ignored in iteration 0); the next layer's are requested into nxt before
-// the compute.  al / be are this iteration's parameter row, pal / pbe the
-// previous iteration's.
+// ignored in iteration 0): a workspace layer's in buf, loaded by the layer
+// before it, which requests them before its own compute; a register layer's
+// in Rr.  al / be are this iteration's parameter row, pal / pbe the previous
+// iteration's.
 template <int LI, bool CAP>
-__device__ __forceinline__ void layer_db(float *Ls, uint32_t *Rbase, int t, uint32_t (&cur)[2 * CfgDB::WMAX],
-                                         uint32_t (&nxt)[2 * CfgDB::WMAX], bool first, const float *al,
+__device__ __forceinline__ void layer_db(float *Ls, uint32_t *Rbase, int t, uint32_t (&buf)[2 * CfgDB::WMAX],
+                                         uint32_t (&Rr)[CfgDB::NREGW], bool first, const float *al,
                                          const float *be, const float *pal, const float *pbe, bool corr,
                                          float rcap) {
     constexpr int D = CfgDB::S.layer[LI].deg, R0 = CfgDB::r_off(LI), W = db_words(LI), W0 = CfgDB::w_off(LI);
-    constexpr bool PACK = D > 3;
+    constexpr bool PACK = D > 3, HBM = db_hbm(LI);
     constexpr int LN = LI + 1 < NLAYERS ? LI + 1 : 0;
     // The parameter rows are read through the constant address space, i.e.
     // with scalar loads: a vector load here would wait (the vector-memory
@@ -1123,32 +1145,39 @@ __device__ __forceinline__ void layer_db(float *Ls, uint32_t *Rbase, int t, uint
             pb[d] = pbec[R0 + d];
         }
     }
-    // unconditional (iteration 0 loads stale words that its next layer
-    // ignores, and a branch would only add waits)
-    load_r_db<LN>(Rbase, t, nxt);
-    check_db<LI, 0, CAP>(Ls, t, cur, first, pa, pb, alv, bev, corr, rcap);
-    check_db<LI, 1, CAP>(Ls, t + CfgDB::THREADS, cur, first, pa, pb, alv, bev, corr, rcap);
-    int tw = t;
-    asm volatile("" : "+v"(tw));
-    uint32_t *Rw = Rbase + tw;
+    if constexpr (HBM) {
+        check_db<LI, 0, CAP>(Ls, t, buf, first, pa, pb, alv, bev, corr, rcap);
+        check_db<LI, 1, CAP>(Ls, t + CfgDB::THREADS, buf, first, pa, pb, alv, bev, corr, rcap);
+        int tw = t;
+        asm volatile("" : "+v"(tw));
+        uint32_t *Rw = Rbase + tw;
 #pragma unroll
-    for (int k = 0; k < W; ++k) {
-        Rw[(2 * (W0 + k)) * CfgDB::THREADS] = cur[2 * k];
-        Rw[(2 * (W0 + k) + 1) * CfgDB::THREADS] = cur[2 * k + 1];
+        for (int k = 0; k < W; ++k) {
+            Rw[(2 * (W0 + k)) * CfgDB::THREADS] = buf[2 * k];
+            Rw[(2 * (W0 + k) + 1) * CfgDB::THREADS] = buf[2 * k + 1];
+        }
+    } else {
+        // unconditional (iteration 0 loads stale words that the next layer
+        // ignores, and a branch would only add waits)
+        if constexpr (db_hbm(LN)) load_r_db<LN>(Rbase, t, buf);
+        constexpr int O = db_reg_off(LI);
+        uint32_t w[2 * CfgDB::WMAX];
+#pragma unroll
+        for (int i = 0; i < 2 * W; ++i) w[i] = Rr[O + i];
+        check_db<LI, 0, CAP>(Ls, t, w, first, pa, pb, alv, bev, corr, rcap);
+        check_db<LI, 1, CAP>(Ls, t + CfgDB::THREADS, w, first, pa, pb, alv, bev, corr, rcap);
+#pragma unroll
+        for (int i = 0; i < 2 * W; ++i) Rr[O + i] = w[i];
     }
     lds_barrier();
 }
 
 template <bool CAP, int... LI>
-__device__ __forceinline__ void iteration_db(float *Ls, uint32_t *Rbase, int t, uint32_t (&b0)[2 * CfgDB::WMAX],
-                                             uint32_t (&b1)[2 * CfgDB::WMAX], bool first, const float *al,
+__device__ __forceinline__ void iteration_db(float *Ls, uint32_t *Rbase, int t, uint32_t (&buf)[2 * CfgDB::WMAX],
+                                             uint32_t (&Rr)[CfgDB::NREGW], bool first, const float *al,
                                              const float *be, const float *pal, const float *pbe, bool corr,
                                              float rcap, std::integer_sequence<int, LI...>) {
-    // layer LI reads buffer LI & 1 and prefetches into the other; the last
-    // layer prefetches layer 0 of the next iteration
-    ((LI & 1 ? layer_db<LI, CAP>(Ls, Rbase, t, b1, b0, first, al, be, pal, pbe, corr, rcap)
-             : layer_db<LI, CAP>(Ls, Rbase, t, b0, b1, first, al, be, pal, pbe, corr, rcap)),
-     ...);
+    (layer_db<LI, CAP>(Ls, Rbase, t, buf, Rr, first, al, be, pal, pbe, corr, rcap), ...);
 }
 
 template <int LI>
@@ -1178,9 +1207,11 @@ __global__ void __launch_bounds__(CfgDB::THREADS) k_d_big(MultiParams P) {
     const float rcap = P.d_rmax;
     uint32_t kstat = 0;
     unsigned long long blocks = 0, fe = 0, its = 0;
-    uint32_t b0[2 * CfgDB::WMAX], b1[2 * CfgDB::WMAX];
+    uint32_t buf[2 * CfgDB::WMAX], Rr[CfgDB::NREGW];
 #pragma unroll
-    for (int i = 0; i < 2 * CfgDB::WMAX; ++i) b0[i] = b1[i] = 0u;
+    for (int i = 0; i < 2 * CfgDB::WMAX; ++i) buf[i] = 0u;
+#pragma unroll
+    for (int i = 0; i < CfgDB::NREGW; ++i) Rr[i] = 0u;
     for (;;) {
         uint32_t cw;
         const uint32_t s = blockIdx.x + kstat * n_wg;
@@ -1210,7 +1241,7 @@ __global__ void __launch_bounds__(CfgDB::THREADS) k_d_big(MultiParams P) {
         while (it < P.max_iter) {
             // the previous iteration's row rebuilds the stored messages
             const int row = d_row(P, it), prow = d_row(P, it > 0 ? it - 1 : 0);
-            iteration_db<CAP>(Ls, Rbase, t, b0, b1, it == 0, P.d_alpha + row * CfgDB::NR, P.d_beta + row * CfgDB::NR,
+            iteration_db<CAP>(Ls, Rbase, t, buf, Rr, it == 0, P.d_alpha + row * CfgDB::NR, P.d_beta + row * CfgDB::NR,
                          P.d_alpha + prow * CfgDB::NR, P.d_beta + prow * CfgDB::NR, corr, rcap,
                          std::make_integer_sequence<int, NLAYERS>{});
             ++it;

@@ -166,9 +166,14 @@ def test_missing_library_fails_loudly(monkeypatch, tmp_path):
 def test_profile_d_arguments_checked_without_gpu():
     lib = L.load()
     h = C.c_void_p()
+    # every AR4JA code has a Profile D kernel: K=16384 r1/2 passes the
+    # argument checks and stops at the device (none here)
     big = L.CodeDesc(L.LDPC_CODE_AR4JA, 16384, 1, 2)
     cfg = L.DecoderCfg(L.LDPC_PROFILE_D, 20, 0.0)
-    assert lib.ldpc_create(C.byref(big), C.byref(cfg), 0, C.byref(h)) == L.LDPC_ERR_UNSUPPORTED
+    rc = lib.ldpc_create(C.byref(big), C.byref(cfg), 0, C.byref(h))
+    assert rc in (L.LDPC_OK, L.LDPC_ERR_NODEV)
+    if rc == L.LDPC_OK:
+        lib.ldpc_destroy(h)
     code = L.CodeDesc(L.LDPC_CODE_AR4JA, 1024, 1, 2)
     cfg_v = L.DecoderCfg(L.LDPC_PROFILE_D, 20, 0.0, 0, 0, 0, 0, L.variant(1, 0))
     assert lib.ldpc_create(C.byref(code), C.byref(cfg_v), 0, C.byref(h)) == L.LDPC_ERR_ARG

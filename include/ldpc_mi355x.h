@@ -63,7 +63,9 @@ enum {
      * fp32 LLRs and messages, r = copysign(min(max(alpha_p g - beta_p, 0),
      * r_max), sign) with g the leave-one-out minimum and alpha / beta per
      * (layer, circulant) from ldpc_set_dmin_params; channel as Profile T.
-     * AR4JA codes with z <= 1024 (all but K=16384 r1/2). */
+     * All nine AR4JA codes; for K=16384 r1/2 ldpc_create also allocates a
+     * message workspace of 144 KiB per workgroup of the launch grid (one
+     * workgroup per CU). */
     LDPC_PROFILE_D = 3
 };
 

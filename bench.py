@@ -288,8 +288,9 @@ def bench_scalar(args) -> None:
         "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "bf16",
         "data": "synthetic: all-zero codeword, BPSK, AWGN from Philox4x32-10 + Box-Muller generated on device",
-        "config": {"workload": "Profile R on matrices/AR4JA_r45_4c_128c_r12.chinn.out (rows in file order, 37 "
-                               "levels), 512 punctured, Eb/N0 2.0 dB, 200 max iterations, 2^20 codewords per step",
+        "config": {"workload": "Profile R on matrices/AR4JA_r45_4c_128c_r12.chinn.out (rows in file order, run "
+                               "in 37 passes of <= 64), 512 punctured, Eb/N0 2.0 dB, 200 max iterations, 2^20 "
+                               "codewords per step",
                    "max_iter": MAX_ITER, "codewords_per_step": CW_PER_STEP},
         "roofline": {"bound": "valu", "achieved": ops / per_launch_s / 1e12, "peak": PEAK_VALU_TOPS,
                      "unit": "Tlane-op/s", "frac": ops / per_launch_s / 1e12 / PEAK_VALU_TOPS, "traffic": None,

@@ -177,6 +177,16 @@ int ldpc_list_variants(const ldpc_code_desc *code, uint32_t *out, int32_t cap);
  * order.  Writes m levels (1-based) and returns the number of levels. */
 int ldpc_row_levels(uint32_t n, uint32_t m, const int32_t *row_ptr, const int32_t *col, int32_t *level);
 
+/* Pass of every row in the Profile R kernel's schedule (needs no GPU; no
+ * reference counterpart: it describes how the sequential row order is run
+ * in parallel).  A row joins a pass of at most `width` rows once every
+ * earlier row sharing a variable with it sits in an earlier pass; among
+ * ready rows the longest remaining dependency chain goes first.  The kernel
+ * uses width 64.  Writes m pass indices (0-based) and returns the number of
+ * passes. */
+int ldpc_row_passes(uint32_t n, uint32_t m, const int32_t *row_ptr, const int32_t *col, int32_t width,
+                    int32_t *pass);
+
 /* Monte Carlo point: generate, decode and tally codewords first_cw.. until
  * max_blocks or target_fe is reached, in launches of chunk_cw codewords.
  * hist (may be NULL) has max_iter + 1 entries: hist[i] = codewords that

@@ -141,6 +141,9 @@ def test_row_levels_rejects_bad_matrices():
     rp2 = (C.c_int32 * 3)(0, 2, 4)
     assert lib.ldpc_row_levels(3, 2, rp2, col, out) == L.LDPC_ERR_ARG   # variable 3 out of range
     assert lib.ldpc_row_levels(4, 2, rp2, col, out) == 1 and list(out) == [1, 1]
+    assert lib.ldpc_row_passes(3, 2, rp2, col, 64, out) == L.LDPC_ERR_ARG  # variable 3 out of range
+    assert lib.ldpc_row_passes(4, 2, rp2, col, 0, out) == L.LDPC_ERR_ARG  # width 0
+    assert lib.ldpc_row_passes(4, 2, rp2, col, 1, out) == 2 and list(out) == [0, 1]
 
 
 def test_profile_t_rejects_clamps_and_matrices():

@@ -87,9 +87,31 @@ def test_r_levels_are_sequentially_equivalent(chinn):
     assert sum(len(x) for x in schedule_r.levels(rows, N)) == M
 
 
+def test_r_passes_are_sequentially_equivalent(chinn):
+    """The kernel's list schedule (passes of <= 64 rows, longest remaining
+    dependency chain first) takes 37 passes for the shipped matrix, the length
+    of its chain; for every variable the rows touching it sit in strictly
+    increasing passes in file order (so no pass holds two rows sharing a
+    variable); libldpc's C++ schedule equals the host one, and at width 32 it
+    packs the 1536 rows into 48 full passes."""
+    from ldpc_mi355x import decoder, schedule_r
+    M, N, rows = chinn
+    ps = schedule_r.passes(rows, N, 64)
+    assert max(ps) + 1 == 37 and np.bincount(ps).max() <= 64
+    assert np.array_equal(decoder.row_passes(rows, N, 64), ps)
+    last = {}
+    for i, r in enumerate(rows):
+        for v in r:
+            assert last.get(v, -1) < ps[i]
+            last[v] = ps[i]
+    p32 = decoder.row_passes(rows, N, 32)
+    assert np.array_equal(p32, schedule_r.passes(rows, N, 32))
+    assert p32.max() + 1 == 48 and np.bincount(p32).min() == 32
+
+
 def test_level_parallel_decode_equals_sequential(chinn, oracle):
-    """Re-ordering rows level by level (what the GPU does) leaves the oracle's
-    sequential Profile R result unchanged."""
+    """Re-ordering rows level by level, or pass by pass as the GPU does,
+    leaves the oracle's sequential Profile R result unchanged."""
     from ldpc_mi355x import schedule_r
     from conftest import channel_params
     M, N, rows = chinn
@@ -100,6 +122,12 @@ def test_level_parallel_decode_equals_sequential(chinn, oracle):
     par, it2, _ = oracle.decode_r(N, [rows[i] for i in order], llr, 40)
     np.testing.assert_array_equal(seq, par)
     np.testing.assert_array_equal(it1, it2)
+    # the kernel's pass order (file order inside a pass)
+    ps = schedule_r.passes(rows, N, 64)
+    order = sorted(range(M), key=lambda i: (ps[i], i))
+    par, it3, _ = oracle.decode_r(N, [rows[i] for i in order], llr, 40)
+    np.testing.assert_array_equal(seq, par)
+    np.testing.assert_array_equal(it1, it3)
 
 
 def test_reference_worker_monte_carlo_mode_matches_fixture_mode(tmp_path):

@@ -14,11 +14,12 @@
 //   syndrome on the raw sign bit, from the third iteration on    (:526-553)
 //   frame error <=> a transmitted sign bit set; bit errors count them
 //                                                                 (:559-577)
-// Rows are decoded in file order, sequential Gauss-Seidel.  The host splits
-// that order into ASAP levels: rows of a level share no variable, and every
-// row's earlier neighbours sit in earlier levels, so processing level by
-// level reads exactly the values the sequential order reads.  A level is cut
-// into passes of up to 64 rows, one row per lane.
+// Rows are decoded in file order, sequential Gauss-Seidel.  The host packs
+// that order into passes of up to 64 rows, one row per lane: a row joins a
+// pass only once every earlier row sharing a variable with it sits in an
+// earlier pass, so the rows of a pass touch disjoint variables and each
+// reads exactly the values the sequential order reads (37 passes for the
+// r1/2 K=1024 matrix, the length of its dependency chain).
 //
 // Layout: one codeword per wave64; the wave's variable LLRs and per-edge
 // messages (bf16) live in LDS; the pass table (one entry per lane and pass:
@@ -75,7 +76,7 @@ template <int MAXD>
 __device__ __forceinline__ RowRaw<MAXD> load_row(const uint4 *table, int pass, int lane) {
     RowRaw<MAXD> r;
 #pragma unroll
-    for (int k = 0; k < MAXD / 6; ++k) r.w[k] = table[(size_t)(pass * 64 + lane) * (MAXD / 6) + k];
+    for (int k = 0; k < MAXD / 6; ++k) r.w[k] = table[(size_t)(pass * R_PASS_ROWS + lane) * (MAXD / 6) + k];
     return r;
 }
 

@@ -34,9 +34,11 @@ hipError_t t12_launch_gen(uint64_t seed, uint64_t first_cw, uint32_t n_cw, float
                           hipStream_t st);
 int t12_schedule(int32_t *out, int cap);
 
-// Profile R (decode_r.hip): level-scheduled rows, one row per lane.
+// Profile R (decode_r.hip): list-scheduled rows, one codeword per wave, one
+// row per lane and pass.
+constexpr int R_PASS_ROWS = 64;
 struct RLaunch {
-    const void *table;  // n_pass * 64 entries of 16 B (maxd <= 6) or 32 B (maxd <= 12)
+    const void *table;  // n_pass * R_PASS_ROWS entries of 16 B (maxd <= 6) or 32 B (maxd <= 12)
     int n_pass, n, n_tx, max_iter;
     int n_edges;  // message slots in LDS: the code's edges plus MAXD - maxd spare
     float r_max, l_post_max, l_max;

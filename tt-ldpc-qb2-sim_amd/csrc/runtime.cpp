@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <queue>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -121,6 +122,56 @@ int levelize(uint32_t n, uint32_t m, const int32_t *rp, const int32_t *col, int3
     return n_levels;
 }
 
+// Rows in passes of at most `width`, for the Profile R kernel.  A row is
+// ready once every earlier row (file order) sharing a variable with it sits
+// in an earlier pass, so the rows of one pass touch disjoint variables and
+// each reads exactly the LLRs the sequential order reads.  Among ready rows
+// the one with the longest chain of dependent rows after it goes first (then
+// file order).  For the r1/2 K=1024 matrix, passes of 64 take 37 steps, the
+// length of the chain (cutting its 37 ASAP levels into passes of 64 takes
+// 38); passes of 32 would take 48, all full.  Writes the pass of every row
+// (0-based) and returns the number of passes.
+int schedule_passes(uint32_t n, uint32_t m, const int32_t *rp, const int32_t *col, int width, int32_t *pass) {
+    std::vector<int32_t> last(n, -1);
+    std::vector<std::vector<uint32_t>> succ(m);
+    std::vector<int32_t> indeg(m, 0);
+    for (uint32_t i = 0; i < m; ++i)
+        for (int32_t e = rp[i]; e < rp[i + 1]; ++e) {
+            const int32_t p = last[col[e]];
+            if (p >= 0) {
+                succ[p].push_back(i);
+                ++indeg[i];
+            }
+            last[col[e]] = (int32_t)i;
+        }
+    std::vector<int32_t> height(m);
+    for (uint32_t k = m; k-- > 0;) {
+        int32_t x = 0;
+        for (uint32_t s : succ[k]) x = std::max(x, height[s]);
+        height[k] = x + 1;
+    }
+    auto later = [&](uint32_t a, uint32_t b) { return height[a] != height[b] ? height[a] < height[b] : a > b; };
+    std::priority_queue<uint32_t, std::vector<uint32_t>, decltype(later)> ready(later);
+    for (uint32_t i = 0; i < m; ++i)
+        if (!indeg[i]) ready.push(i);
+    int n_pass = 0;
+    std::vector<uint32_t> cur;
+    while (!ready.empty()) {
+        cur.clear();
+        while (!ready.empty() && (int)cur.size() < width) {
+            cur.push_back(ready.top());
+            ready.pop();
+        }
+        for (uint32_t r : cur) {
+            pass[r] = n_pass;
+            for (uint32_t s : succ[r])
+                if (--indeg[s] == 0) ready.push(s);
+        }
+        ++n_pass;
+    }
+    return n_pass;
+}
+
 int validate_rows(uint32_t n, uint32_t m, const int32_t *rp, const int32_t *col, std::string &why) {
     if (!rp || !col || n == 0 || m == 0) {
         why = "matrix: empty or null arrays";
@@ -145,7 +196,7 @@ int validate_rows(uint32_t n, uint32_t m, const int32_t *rp, const int32_t *col,
     return LDPC_OK;
 }
 
-// Profile R set-up: levels -> passes of <= 64 rows -> device table.
+// Profile R set-up: rows -> passes of <= R_PASS_ROWS rows -> device table.
 int setup_profile_r(ldpc_ctx *c, const std::vector<int32_t> &rp, const std::vector<int32_t> &col, uint32_t n,
                     uint32_t m, uint32_t np, uint32_t k_info) {
     std::string why;
@@ -177,29 +228,20 @@ int setup_profile_r(ldpc_ctx *c, const std::vector<int32_t> &rp, const std::vect
             }
         r_off[last] = off;
     }
-    std::vector<int32_t> level(m);
+    std::vector<int32_t> level(m), pass(m);
     const int n_levels = levelize(n, m, rp.data(), col.data(), level.data());
-    const int words = kmaxd == 6 ? 4 : 8;  // uint32 per lane entry
-    std::vector<std::vector<uint32_t>> by_level(n_levels + 1);
-    for (uint32_t i = 0; i < m; ++i) by_level[level[i]].push_back(i);
-    std::vector<uint32_t> table;
-    int n_pass = 0;
-    for (int l = 1; l <= n_levels; ++l) {
-        const auto &rows = by_level[l];
-        for (size_t off = 0; off < rows.size(); off += 64) {
-            const size_t base = table.size();
-            table.resize(base + 64 * (size_t)words, 0u);
-            for (size_t k = 0; k < 64 && off + k < rows.size(); ++k) {
-                const uint32_t r = rows[off + k];
-                uint32_t *e = &table[base + k * words];
-                const uint32_t deg = (uint32_t)(rp[r + 1] - rp[r]);
-                e[0] = deg | (r_off[r] << 4);
-                // variables 0..5 in words 1..3, 6..11 in words 4..6
-                for (uint32_t d = 0; d < deg; ++d)
-                    e[1 + d / 2] |= ((uint32_t)col[rp[r] + d] & 0xFFFFu) << (16 * (d & 1));
-            }
-            ++n_pass;
-        }
+    const int n_pass = schedule_passes(n, m, rp.data(), col.data(), ldpc::R_PASS_ROWS, pass.data());
+    const int words = kmaxd == 6 ? 4 : 8;  // uint32 per row entry
+    // entry (p, k): the k-th row of pass p (file order inside a pass); unused
+    // entries stay zero (degree 0)
+    std::vector<uint32_t> table((size_t)n_pass * ldpc::R_PASS_ROWS * words, 0u);
+    std::vector<int> fill(n_pass, 0);
+    for (uint32_t r = 0; r < m; ++r) {
+        uint32_t *e = &table[((size_t)pass[r] * ldpc::R_PASS_ROWS + fill[pass[r]]++) * words];
+        const uint32_t deg = (uint32_t)(rp[r + 1] - rp[r]);
+        e[0] = deg | (r_off[r] << 4);
+        // variables 0..5 in words 1..3, 6..11 in words 4..6
+        for (uint32_t d = 0; d < deg; ++d) e[1 + d / 2] |= ((uint32_t)col[rp[r] + d] & 0xFFFFu) << (16 * (d & 1));
     }
     HIP_TRY(c, hipMalloc(&c->d_table, table.size() * sizeof(uint32_t)));
     HIP_TRY(c, hipMemcpy(c->d_table, table.data(), table.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
@@ -306,6 +348,13 @@ int ldpc_row_levels(uint32_t n, uint32_t m, const int32_t *row_ptr, const int32_
     std::string why;
     if (!level || validate_rows(n, m, row_ptr, col, why)) return LDPC_ERR_ARG;
     return levelize(n, m, row_ptr, col, level);
+}
+
+int ldpc_row_passes(uint32_t n, uint32_t m, const int32_t *row_ptr, const int32_t *col, int32_t width,
+                    int32_t *pass) {
+    std::string why;
+    if (!pass || width < 1 || validate_rows(n, m, row_ptr, col, why)) return LDPC_ERR_ARG;
+    return schedule_passes(n, m, row_ptr, col, width, pass);
 }
 
 int ldpc_create(const ldpc_code_desc *code, const ldpc_decoder_cfg *cfg, int device, ldpc_ctx **out) {

@@ -70,6 +70,19 @@ def row_levels(rows, n):
     return out
 
 
+def row_passes(rows, n, width=64):
+    """Pass of every row (0-based) in libldpc's Profile R schedule."""
+    lib = L.load()
+    rp = np.zeros(len(rows) + 1, dtype=np.int32)
+    rp[1:] = np.cumsum([len(r) for r in rows])
+    col = np.concatenate([np.asarray(r, dtype=np.int32) for r in rows]).astype(np.int32)
+    out = np.zeros(len(rows), dtype=np.int32)
+    npass = lib.ldpc_row_passes(n, len(rows), rp.ctypes.data, col.ctypes.data, width, out.ctypes.data)
+    if npass < 0:
+        raise L.LdpcError(npass, "invalid matrix")
+    return out
+
+
 def ar4ja_desc(k_info: int, rate: str) -> L.CodeDesc:
     num, den = RATE_FRACTIONS[rate]
     return L.CodeDesc(L.LDPC_CODE_AR4JA, k_info, num, den)

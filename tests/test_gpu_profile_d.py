@@ -2,7 +2,10 @@
 decode_multi.hip) against the oracle, bit for bit: final LLRs (bf16),
 iterations and convergence of channel dumps; Monte Carlo counters and
 histograms; the channel is the Profile T stream.  K=16384 r1/2 has its own
-kernel (messages in an HBM workspace, L in all of the LDS)."""
+kernel (L in all of the LDS; messages rebuilt from three words per check,
+half of the layers' words in registers and half in an HBM workspace), so
+its cases cover per-iteration parameter rows (the rebuild uses the previous
+iteration's row), the correction and the cap."""
 import numpy as np
 import pytest
 

@@ -45,7 +45,7 @@ struct ldpc_ctx {
     // trapping-set capture (ldpc_set_capture)
     uint32_t *d_cap_count = nullptr;
     unsigned long long *d_cap_ids = nullptr;
-    void *d_work = nullptr;  // per-workgroup workspace of the Profile D kernel for z = 2048
+    void *d_work = nullptr;  // per-workgroup message workspace (Profile D, K=16384 r1/2)
     uint32_t cap_max = 0;
     // Profile D parameters on the device
     float *d_alpha = nullptr, *d_beta = nullptr;

@@ -414,8 +414,8 @@ def main():
         copy_calibration(args.copy_calibration)
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py: this build measures one GPU only (the decode path shards by codeword id across GPUs "
-              "but no multi-GPU figure is produced); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU only (the decode path shards by codeword id across GPUs, but no "
+              "multi-GPU figure is produced); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     if args.workload == "batch9":

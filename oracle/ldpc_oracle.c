@@ -1,7 +1,7 @@
 /*
  * ldpc_oracle.c -- CPU oracle for the Monte Carlo LDPC hot path.
  *
- * TEST INFRASTRUCTURE ONLY.  This file is the checker that the HIP product
+ * TEST INFRASTRUCTURE ONLY.  This file is the oracle that the HIP product
  * path is compared against; it is loaded only by tests/, by the smoke() entry
  * and by bench.py's cpu_baseline leg.  Product code never links or calls it.
  *

@@ -976,6 +976,12 @@ constexpr bool db_hbm_apart() {
     return true;
 }
 static_assert(db_hbm_apart(), "workspace layers must not be adjacent");
+constexpr bool db_words_fit() {
+    for (int i = 0; i < NLAYERS; ++i)
+        if (db_words(i) > 3 || SDB.layer[i].deg > 6) return false;
+    return true;
+}
+static_assert(db_words_fit(), "three words per check, degrees up to 6");
 
 struct CfgDB {
     static constexpr int C = 2, ORD = 0;
@@ -990,7 +996,7 @@ struct CfgDB {
     static constexpr int NREGW = db_reg_off(NLAYERS);    // register-resident words per thread
     static constexpr int CTL = 2 * NW * THREADS;         // control words: flag[2], queue broadcast
     static constexpr int WORK_DWORDS = CTL + 64;         // per workgroup (control padded to 256 B)
-    static constexpr int DMAX = 6, WMAX = 3;
+    static constexpr int WMAX = 3;  // workspace / register words per check and layer (at most)
     static_assert(Z == 2 * THREADS && N * 4 == 163840, "K = 16384 r1/2 layout");
 };
 
@@ -1026,6 +1032,19 @@ __device__ __forceinline__ void load_r_db(const uint32_t *Rbase, int t, uint32_t
     for (int k = 0; k < W; ++k) {
         buf[2 * k] = Rw[(2 * (W0 + k)) * CfgDB::THREADS];
         buf[2 * k + 1] = Rw[(2 * (W0 + k) + 1) * CfgDB::THREADS];
+    }
+}
+
+// Stores of layer LI's words from buf (the layout of load_r_db).
+template <int LI>
+__device__ __forceinline__ void store_r_db(uint32_t *Rbase, int t, const uint32_t (&buf)[2 * CfgDB::WMAX]) {
+    constexpr int W = db_words(LI), W0 = CfgDB::w_off(LI);
+    asm volatile("" : "+v"(t));
+    uint32_t *Rw = Rbase + t;
+#pragma unroll
+    for (int k = 0; k < W; ++k) {
+        Rw[(2 * (W0 + k)) * CfgDB::THREADS] = buf[2 * k];
+        Rw[(2 * (W0 + k) + 1) * CfgDB::THREADS] = buf[2 * k + 1];
     }
 }
 
@@ -1121,7 +1140,7 @@ __device__ __forceinline__ void layer_db(float *Ls, uint32_t *Rbase, int t, uint
                                          uint32_t (&Rr)[CfgDB::NREGW], bool first, const float *al,
                                          const float *be, const float *pal, const float *pbe, bool corr,
                                          float rcap) {
-    constexpr int D = CfgDB::S.layer[LI].deg, R0 = CfgDB::r_off(LI), W = db_words(LI), W0 = CfgDB::w_off(LI);
+    constexpr int D = CfgDB::S.layer[LI].deg, R0 = CfgDB::r_off(LI), W = db_words(LI);
     constexpr bool PACK = D > 3, HBM = db_hbm(LI);
     constexpr int LN = LI + 1 < NLAYERS ? LI + 1 : 0;
     // The parameter rows are read through the constant address space, i.e.
@@ -1148,14 +1167,7 @@ __device__ __forceinline__ void layer_db(float *Ls, uint32_t *Rbase, int t, uint
     if constexpr (HBM) {
         check_db<LI, 0, CAP>(Ls, t, buf, first, pa, pb, alv, bev, corr, rcap);
         check_db<LI, 1, CAP>(Ls, t + CfgDB::THREADS, buf, first, pa, pb, alv, bev, corr, rcap);
-        int tw = t;
-        asm volatile("" : "+v"(tw));
-        uint32_t *Rw = Rbase + tw;
-#pragma unroll
-        for (int k = 0; k < W; ++k) {
-            Rw[(2 * (W0 + k)) * CfgDB::THREADS] = buf[2 * k];
-            Rw[(2 * (W0 + k) + 1) * CfgDB::THREADS] = buf[2 * k + 1];
-        }
+        store_r_db<LI>(Rbase, t, buf);
     } else {
         // unconditional (iteration 0 loads stale words that the next layer
         // ignores, and a branch would only add waits)

@@ -36,3 +36,22 @@ def test_gpus_above_one_exit_with_a_message():
     import subprocess
     p = subprocess.run([sys.executable, bench.__file__, "--gpus", "2"], capture_output=True, text=True, timeout=60)
     assert p.returncode == 2 and "one GPU only" in p.stderr and p.stdout == ""
+
+
+def test_floor_comparisons_quoted_against_the_reference():
+    """The error-floor comparisons the profiles quote, with the same exact test:
+    17 frame errors in 2e10 (Profile T) against the silicon run's 61 in
+    40,000,004,320 (README.md:120) -> p = 0.031; 11 in 5e9 (Profile R) against
+    the scalar engine's 11 in 4,667,520,000 (README.md:121) -> p = 1.0."""
+    pytest.importorskip("scipy")
+    t = bench.fer_match(17, 20_000_000_000, {"blocks": 40_000_004_320, "frame_errors": 61})
+    assert round(t["p_value"], 3) == 0.031 and t["consistent_95"] is False
+    r = bench.fer_match(11, 5_000_000_000, {"blocks": 4_667_520_000, "frame_errors": 11})
+    assert r["p_value"] > 0.99 and r["consistent_95"]
+
+
+def test_floor_profile_option_is_validated():
+    import subprocess
+    p = subprocess.run([sys.executable, bench.__file__, "--workload", "floor", "--floor-profile", "X"],
+                       capture_output=True, text=True, timeout=60)
+    assert p.returncode == 2 and "--floor-profile" in p.stderr and p.stdout == ""

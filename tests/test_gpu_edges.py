@@ -1,6 +1,6 @@
 """Edge cases through the C-ABI on the GPU, for every kernel family
 (specialised Profile T, the generic nine-code kernel in wave and group mode,
-Profile R, Profile D): empty calls return without work and leave the
+Profile R, Profile D and its K=16384 kernel): empty calls return without work and leave the
 codeword cursor where it was; ragged batch sizes (a single codeword, a
 partly filled wave of codeword slots, one more than a multiple of the slots)
 decode bit for bit like the oracle; a Monte Carlo run split at arbitrary
@@ -25,19 +25,22 @@ def _case(msg):
 def _engines():
     """(name, factory) for one engine of every kernel family."""
     _, n, rows = codes.read_chinn_out(CHINN)
-    p = 60  # parameters of r1/2 K=1024 (sum of the layer degrees)
+    p = 60  # parameters of r1/2 (sum of the layer degrees), K=1024 and 16384 alike
     a, b = np.full((1, p), 0.85, np.float32), np.full((1, p), 0.1, np.float32)
 
-    def prof_d():
-        e = decoder.Engine(1024, "1/2", max_iter=30, profile="D")
-        e.set_dmin_params(a, b)
-        return e
+    def prof_d(k):
+        def make():
+            e = decoder.Engine(k, "1/2", max_iter=30, profile="D")
+            e.set_dmin_params(a, b)
+            return e
+        return make
     return [
         ("T specialised r1/2 K=1024", lambda: decoder.Engine(1024, "1/2", max_iter=30)),
         ("T generic wave K=1024 r4/5", lambda: decoder.Engine(1024, "4/5", max_iter=30)),
         ("T generic group K=4096 r2/3", lambda: decoder.Engine(4096, "2/3", max_iter=30)),
         ("R .chinn.out", lambda: decoder.Engine(profile="R", matrix=(rows, n), n_punct=512, max_iter=30)),
-        ("D r1/2 K=1024", prof_d),
+        ("D r1/2 K=1024", prof_d(1024)),
+        ("D r1/2 K=16384", prof_d(16384)),
     ]
 
 
@@ -59,7 +62,7 @@ def _oracle_decode(oracle, name, eng, llr):
         return oracle.decode_r(n, rows, llr, 30)
     if name.startswith("D"):
         import dmin_oracle as DO
-        code = codes.ar4ja(1024, "1/2")
+        code = codes.ar4ja(eng.k_info, "1/2")
         p = DO.n_params(code)
         return DO.decode_d(code, llr, np.full((1, p), 0.85, np.float32), np.full((1, p), 0.1, np.float32), 30)
     return oracle.decode_t(codes.ar4ja(eng.k_info, eng.rate), llr, 30, 0.9375)
@@ -70,7 +73,7 @@ def test_ragged_batch_sizes_decode_like_the_oracle(oracle, name, make):
     with make() as eng:
         for n_cw in (1, 3, 65):
             _case(f"ragged decode: {name} n_cw={n_cw}")
-            ebn0 = 1.6 if eng.rate == "1/2" else (2.2 if eng.rate == "2/3" else 3.0)
+            ebn0 = 1.0 if eng.k_info == 16384 else (1.6 if eng.rate == "1/2" else (2.2 if eng.rate == "2/3" else 3.0))
             llr = eng.gen_channel(0xED6E + n_cw, 17, n_cw, ebn0)
             out, it, ok = eng.decode_llrs(llr)
             e_out, e_it, e_ok = _oracle_decode(oracle, name, eng, llr)
@@ -82,7 +85,7 @@ def test_ragged_batch_sizes_decode_like_the_oracle(oracle, name, make):
 @pytest.mark.parametrize("name,make", [pytest.param(n, f, id=n) for n, f in _engines()])
 def test_monte_carlo_split_at_odd_points_adds_up(name, make):
     with make() as eng:
-        ebn0 = 1.4 if eng.rate == "1/2" else (2.2 if eng.rate == "2/3" else 3.0)
+        ebn0 = 0.9 if eng.k_info == 16384 else (1.4 if eng.rate == "1/2" else (2.2 if eng.rate == "2/3" else 3.0))
         _case(f"split Monte Carlo: {name}")
         whole = eng.run_point(ebn0, 3001, seed=99, first_cw=10, chunk_cw=3001)
         acc = None

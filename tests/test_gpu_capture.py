@@ -72,6 +72,27 @@ def test_profile_r_and_d_capture(oracle):
     np.testing.assert_array_equal(ids, exp)
 
 
+def test_profile_d_k16384_capture():
+    """The K=16384 Profile D kernel records the ids of its failing codewords:
+    over 320 codewords (static split and queue) they equal those that
+    decode_llrs leaves unconverged on the same channel (decode arithmetic the
+    Profile D tests pin to the oracle)."""
+    import dmin_oracle as DO
+    code = codes.ar4ja(16384, "1/2")
+    P = DO.n_params(code)
+    a, b = np.full((1, P), 0.85, np.float32), np.full((1, P), 0.05, np.float32)
+    print("[gpu-case] Profile D K=16384 capture, 320 codewords at 1.0 dB", flush=True)
+    with decoder.Engine(16384, "1/2", max_iter=20, profile="D") as eng:
+        eng.set_dmin_params(a, b, True)
+        eng.set_capture(4096)
+        r = eng.run_point(1.0, 320, seed=13, chunk_cw=320)
+        ids, seen = eng.captures()
+        _, _, ok = eng.decode_llrs(eng.gen_channel(13, 0, 320, 1.0))
+    exp = np.nonzero(ok == 0)[0]
+    assert seen == r.frame_errors == len(exp) > 0
+    np.testing.assert_array_equal(ids, exp)
+
+
 def test_trap_dumps_replay(tmp_path):
     with decoder.Engine(1024, "1/2", max_iter=20) as eng:
         eng.set_capture(16)

@@ -25,7 +25,7 @@ def params(code, G, seed):
 
 
 @pytest.mark.parametrize("k,rate,ebn0", CASES)
-@pytest.mark.parametrize("G,corr,r_max", [(1, False, 0.0), (8, True, 0.0), (1, False, 6.0)])
+@pytest.mark.parametrize("G,corr,r_max", [(1, False, 0.0), (8, True, 0.0), (1, False, 6.0), (8, True, 6.0)])
 def test_decode_matches_oracle(oracle, k, rate, ebn0, G, corr, r_max):
     code = codes.ar4ja(k, rate)
     mu, sg = channel_params(ebn0, code.k / code.n_tx)

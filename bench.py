@@ -315,7 +315,9 @@ def bench_floor(args) -> None:
     else:
         eng = decoder.Engine(1024, "1/2", max_iter=MAX_ITER, beta=BETA)
     eng.set_capture(64)  # ids of the failing codewords (replayable: seed + id regenerate the channel)
-    n, acc, first = args.blocks, None, 0
+    # --floor-first continues an earlier run: codeword ids are the channel's
+    # counter, so segments [0, a) and [a, a + b) add up to one run of a + b
+    n, acc, first = args.blocks, None, args.floor_first
     t0 = time.perf_counter()
     while (acc.blocks if acc else 0) < n:
         k = min(100 * CW_PER_STEP, n - (acc.blocks if acc else 0))
@@ -335,6 +337,7 @@ def bench_floor(args) -> None:
            "latest_converged_iteration": latest, "wall_s": wall, "kernel_gsps": acc.kernel_gsps,
            "higher_is_better": True, "dtype": "bf16", "data": "synthetic (on-device Philox/Box-Muller)",
            "reference_published_fer": 2.36e-9 if args.floor_profile == "R" else 1.53e-9, "seed": 0x5EED2026 ^ 0x2500,
+           "first_codeword_id": args.floor_first,
            "failing_codeword_ids": [int(i) for i in ids]}
     print(json.dumps(out), flush=True)
     eng.close()
@@ -414,6 +417,8 @@ def main():
                     help="r12k1024: the headline AR4JA r1/2 K=1024 point; scalar: Profile R on the .chinn.out; "
                          "batch9: the nine-code mixed batch; floor: 1e9 codewords at 2.5 dB")
     ap.add_argument("--blocks", type=int, default=10 ** 9, help="codewords of the floor workload")
+    ap.add_argument("--floor-first", type=int, default=0,
+                    help="first codeword id of the floor workload (continue an earlier run of the same seed)")
     ap.add_argument("--floor-profile", choices=["T", "R"], default="T",
                     help="decoder of the floor workload: T (golden arithmetic; reference silicon 1.53e-9) or R "
                          "(scalar_5w arithmetic on the .chinn.out; reference 2.36e-9)")

@@ -42,14 +42,14 @@ def test_floor_comparisons_quoted_against_the_reference():
     """The error-floor comparisons the profiles quote, with the same exact test:
     34 frame errors in 4e10 (Profile T) against the silicon run's 61 in
     40,000,004,320 (README.md:120) -> p = 0.0073, and 17 in the first 2e10 ->
-    p = 0.031; 11 in 5e9 (Profile R) against the scalar engine's 11 in
+    p = 0.031; 24 in 1e10 (Profile R) against the scalar engine's 11 in
     4,667,520,000 (README.md:121) -> p = 1.0."""
     pytest.importorskip("scipy")
     t = bench.fer_match(34, 40_000_000_000, {"blocks": 40_000_004_320, "frame_errors": 61})
     assert round(t["p_value"], 4) == 0.0073 and t["consistent_95"] is False
     t1 = bench.fer_match(17, 20_000_000_000, {"blocks": 40_000_004_320, "frame_errors": 61})
     assert round(t1["p_value"], 3) == 0.031
-    r = bench.fer_match(11, 5_000_000_000, {"blocks": 4_667_520_000, "frame_errors": 11})
+    r = bench.fer_match(24, 10_000_000_000, {"blocks": 4_667_520_000, "frame_errors": 11})
     assert r["p_value"] > 0.99 and r["consistent_95"]
 
 

@@ -574,6 +574,58 @@ EXPORT void orc_gen_channel(uint64_t seed, uint64_t first_cw, int n_cw, float mu
 /* Bit errors: info bits k < k_info with bf16 <= 0 of unconverged codewords  */
 /* (stepper:801-809).                                                         */
 /* ------------------------------------------------------------------------- */
+/* Tally of one decoded codeword (stepper:799-815): it = iteration of
+ * convergence (1..max_iter) or 0 if the syndrome never cleared.  A codeword
+ * still active after max_iter is a frame error, counts max_iter iterations
+ * and adds its info bits k < k_info with bf16 <= 0.0f to the bit errors;
+ * converged codewords add nothing but their iteration count.               */
+static void tally_t(const uint16_t *L, int it, int k_info, int max_iter, uint64_t *fe, uint64_t *be, uint64_t *its,
+                    uint64_t *h) {
+    if (it) {
+        *its += (uint64_t)it;
+        h[it] += 1;
+    } else {
+        *its += (uint64_t)max_iter;
+        h[0] += 1;
+        *fe += 1;
+        for (int k = 0; k < k_info; ++k) *be += (uint64_t)nonpos(L[k]);
+    }
+}
+
+/* Decode caller-supplied channel LLRs (n_cw x n bf16, codeword-major) and
+ * tally them exactly as orc_run_point_t tallies its own channel: the golden
+ * Monte Carlo accounting on any channel, e.g. the reference's own draws. */
+EXPORT int orc_run_llrs_t(int n, int m, const int32_t *rp, const int32_t *col, const int32_t *sched, int k_info,
+                          int n_cw, const uint16_t *llr, int max_iter, float beta, int threads, uint64_t *counters,
+                          uint64_t *hist) {
+    if (check_degrees_t(m, rp)) return -1;
+    uint64_t fe = 0, be = 0, its = 0;
+    memset(hist, 0, sizeof(uint64_t) * (size_t)(max_iter + 1));
+#ifdef _OPENMP
+    if (threads <= 0) threads = omp_get_max_threads();
+#pragma omp parallel num_threads(threads) reduction(+ : fe, be, its)
+#endif
+    {
+        uint16_t *L = (uint16_t *)malloc(sizeof(uint16_t) * (size_t)n);
+        uint16_t *R = (uint16_t *)malloc(sizeof(uint16_t) * (size_t)rp[m]);
+        uint64_t *h = (uint64_t *)calloc((size_t)max_iter + 1, sizeof(uint64_t));
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 4)
+#endif
+        for (int c = 0; c < n_cw; ++c) {
+            memcpy(L, llr + (size_t)c * n, sizeof(uint16_t) * n);
+            tally_t(L, decode_one_t(L, R, m, rp, col, sched, max_iter, beta), k_info, max_iter, &fe, &be, &its, h);
+        }
+#ifdef _OPENMP
+#pragma omp critical
+#endif
+        for (int i = 0; i <= max_iter; ++i) hist[i] += h[i];
+        free(L); free(R); free(h);
+    }
+    counters[0] = (uint64_t)n_cw; counters[1] = fe; counters[2] = be; counters[3] = its;
+    return 0;
+}
+
 EXPORT int orc_run_point_t(int n, int m, const int32_t *rp, const int32_t *col, const int32_t *sched,
                            int n_tx, int k_info, uint64_t seed, uint64_t first_cw, uint64_t n_cw,
                            float mu, float sigma, int max_iter, float beta, int threads,
@@ -595,17 +647,8 @@ EXPORT int orc_run_point_t(int n, int m, const int32_t *rp, const int32_t *col, 
 #endif
         for (int64_t c = 0; c < (int64_t)n_cw; ++c) {
             gen_one(key, first_cw + (uint64_t)c, mu, sigma, n_tx, n, L);
-            int it = decode_one_t(L, R, m, rp, col, sched, max_iter, beta);
             blocks += 1;
-            if (it) {
-                its += (uint64_t)it;
-                h[it] += 1;
-            } else {
-                its += (uint64_t)max_iter;
-                h[0] += 1;
-                fe += 1;
-                for (int k = 0; k < k_info; ++k) be += (uint64_t)nonpos(L[k]);
-            }
+            tally_t(L, decode_one_t(L, R, m, rp, col, sched, max_iter, beta), k_info, max_iter, &fe, &be, &its, h);
         }
 #ifdef _OPENMP
 #pragma omp critical

@@ -64,6 +64,9 @@ def lib():
         L.orc_run_point_t.argtypes = [C.c_int, C.c_int, i32p, i32p, i32p, C.c_int, C.c_int, C.c_uint64,
                                       C.c_uint64, C.c_uint64, C.c_float, C.c_float, C.c_int, C.c_float, C.c_int,
                                       u64p, u64p]
+        L.orc_run_llrs_t.restype = C.c_int
+        L.orc_run_llrs_t.argtypes = [C.c_int, C.c_int, i32p, i32p, i32p, C.c_int, C.c_int, u16p, C.c_int, C.c_float,
+                                     C.c_int, u64p, u64p]
         L.orc_decode_r.restype = C.c_int
         L.orc_decode_r.argtypes = [C.c_int, C.c_int, i32p, i32p, C.c_int, u16p, C.c_int, C.c_int, C.c_float, C.c_float,
                                    u16p, u32p, u8p, C.c_int]
@@ -194,6 +197,21 @@ def run_point_t(code, seed: int, first_cw: int, n_cw: int, mu: float, sigma: flo
     hist = np.zeros(max_iter + 1, dtype=np.uint64)
     rc = lib().orc_run_point_t(code.n, code.m, rp, col, sched, code.n_tx, code.k, seed, first_cw, n_cw, mu, sigma,
                                max_iter, beta, threads, cnt, hist)
+    if rc:
+        raise ValueError("oracle Profile T supports check degree 3 and even degrees 4..18")
+    return dict(blocks=int(cnt[0]), frame_errors=int(cnt[1]), bit_errors=int(cnt[2]), iter_sum=int(cnt[3]),
+                hist=hist)
+
+
+def run_llrs_t(code, llr: np.ndarray, max_iter: int = 200, beta: float = 0.9375, threads: int = 0):
+    """Decode given channel LLRs [n_cw, n] and tally them with run_point_t's
+    rule.  Returns dict(blocks, frame_errors, bit_errors, iter_sum, hist)."""
+    llr = np.ascontiguousarray(llr, dtype=np.uint16).reshape(-1, code.n)
+    rp, col, sched = code_arrays(code)
+    cnt = np.zeros(4, dtype=np.uint64)
+    hist = np.zeros(max_iter + 1, dtype=np.uint64)
+    rc = lib().orc_run_llrs_t(code.n, code.m, rp, col, sched, code.k, llr.shape[0], llr, max_iter, beta, threads,
+                              cnt, hist)
     if rc:
         raise ValueError("oracle Profile T supports check degree 3 and even degrees 4..18")
     return dict(blocks=int(cnt[0]), frame_errors=int(cnt[1]), bit_errors=int(cnt[2]), iter_sum=int(cnt[3]),

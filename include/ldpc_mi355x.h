@@ -81,8 +81,9 @@ typedef struct ldpc_code_desc {
     uint32_t rate_num; /* AR4JA: 1, 2 or 4 */
     uint32_t rate_den; /* AR4JA: 2, 3 or 5 */
     /* LDPC_CODE_MATRIX: m rows in decoding order, each row's variables in
-     * order (row_ptr[m+1], col[row_ptr[m]], 0-based); the last n_punct
-     * variables are punctured.  Copied at ldpc_create. */
+     * order (row_ptr[m+1], col[row_ptr[m]], 0-based; a row may not list a
+     * variable twice); the last n_punct variables are punctured.  Copied at
+     * ldpc_create. */
     uint32_t n, m, n_punct, reserved;
     const int32_t *row_ptr;
     const int32_t *col;

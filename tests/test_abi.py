@@ -146,6 +146,28 @@ def test_row_levels_rejects_bad_matrices():
     assert lib.ldpc_row_passes(4, 2, rp2, col, 1, out) == 2 and list(out) == [0, 1]
 
 
+def test_repeated_column_is_refused_and_never_deadlocks_the_schedule():
+    """A row listing one variable twice ([[0, 1, 1], [1, 2], [2, 3], [0, 3]])
+    once made the list scheduler add an edge from the row to itself, so that
+    no row was ever ready and the pass table was built empty.  libldpc refuses
+    the matrix; the Python mirror, which takes any rows, schedules it."""
+    lib = L.load()
+    rows = [[0, 1, 1], [1, 2], [2, 3], [0, 3]]
+    rp = (C.c_int32 * 5)(0, 3, 5, 7, 9)
+    col = (C.c_int32 * 9)(*[v for r in rows for v in r])
+    out = (C.c_int32 * 4)()
+    assert lib.ldpc_row_passes(4, 4, rp, col, 64, out) == L.LDPC_ERR_ARG
+    assert lib.ldpc_row_levels(4, 4, rp, col, out) == L.LDPC_ERR_ARG
+    from ldpc_mi355x import schedule_r
+    assert schedule_r.passes(rows, 4, 64) == [0, 1, 2, 3]
+    # the same rows without the repeat are accepted and chain one row per pass
+    rows_ok = [[0, 1], [1, 2], [2, 3], [0, 3]]
+    rp_ok = (C.c_int32 * 5)(0, 2, 4, 6, 8)
+    col_ok = (C.c_int32 * 8)(*[v for r in rows_ok for v in r])
+    assert lib.ldpc_row_passes(4, 4, rp_ok, col_ok, 64, out) == 4 and list(out) == [0, 1, 2, 3]
+    assert schedule_r.passes(rows_ok, 4, 64) == [0, 1, 2, 3]
+
+
 def test_profile_t_rejects_clamps_and_matrices():
     lib = L.load()
     h = C.c_void_p()

@@ -138,7 +138,7 @@ int schedule_passes(uint32_t n, uint32_t m, const int32_t *rp, const int32_t *co
     for (uint32_t i = 0; i < m; ++i)
         for (int32_t e = rp[i]; e < rp[i + 1]; ++e) {
             const int32_t p = last[col[e]];
-            if (p >= 0) {
+            if (p >= 0 && p != (int32_t)i) {  // a row listing a variable twice has no edge to itself
                 succ[p].push_back(i);
                 ++indeg[i];
             }
@@ -169,6 +169,8 @@ int schedule_passes(uint32_t n, uint32_t m, const int32_t *rp, const int32_t *co
         }
         ++n_pass;
     }
+    for (uint32_t i = 0; i < m; ++i)
+        if (indeg[i]) return -1;  // unreachable for a dependency graph built in file order
     return n_pass;
 }
 
@@ -187,11 +189,20 @@ int validate_rows(uint32_t n, uint32_t m, const int32_t *rp, const int32_t *col,
             why = "matrix: check degree must be 1..12 (ldpc_sim_scalar.cpp:34)";
             return LDPC_ERR_ARG;
         }
-        for (int32_t e = rp[i]; e < rp[i + 1]; ++e)
+        for (int32_t e = rp[i]; e < rp[i + 1]; ++e) {
             if (col[e] < 0 || (uint32_t)col[e] >= n) {
                 why = "matrix: variable index out of range";
                 return LDPC_ERR_ARG;
             }
+            // A repeated (row, column) entry has no single meaning (over GF(2) it
+            // cancels; as two edges it double-counts a message), so it is refused.
+            for (int32_t f = rp[i]; f < e; ++f)
+                if (col[f] == col[e]) {
+                    why = "matrix: row " + std::to_string(i + 1) + " lists variable " + std::to_string(col[e] + 1) +
+                          " twice";
+                    return LDPC_ERR_ARG;
+                }
+        }
     }
     return LDPC_OK;
 }
@@ -231,6 +242,7 @@ int setup_profile_r(ldpc_ctx *c, const std::vector<int32_t> &rp, const std::vect
     std::vector<int32_t> level(m), pass(m);
     const int n_levels = levelize(n, m, rp.data(), col.data(), level.data());
     const int n_pass = schedule_passes(n, m, rp.data(), col.data(), ldpc::R_PASS_ROWS, pass.data());
+    if (n_pass <= 0) return fail(nullptr, LDPC_ERR_ARG, "Profile R: rows could not be scheduled");
     const int words = kmaxd == 6 ? 4 : 8;  // uint32 per row entry
     // entry (p, k): the k-th row of pass p (file order inside a pass); unused
     // entries stay zero (degree 0)

@@ -45,7 +45,7 @@ def passes(rows: Sequence[Sequence[int]], n: int, width: int = 64) -> List[int]:
     indeg = [0] * m
     for i, r in enumerate(rows):
         for v in r:
-            if last[v] >= 0:
+            if last[v] >= 0 and last[v] != i:  # a repeated column is no edge to itself
                 succ[last[v]].append(i)
                 indeg[i] += 1
             last[v] = i
@@ -65,4 +65,6 @@ def passes(rows: Sequence[Sequence[int]], n: int, width: int = 64) -> List[int]:
                 if indeg[s] == 0:
                     heapq.heappush(ready, (-height[s], s))
         p += 1
+    if any(indeg):
+        raise ValueError("rows could not be scheduled")
     return out

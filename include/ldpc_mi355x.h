@@ -155,8 +155,9 @@ int ldpc_create(const ldpc_code_desc *code, const ldpc_decoder_cfg *cfg, int dev
 
 void ldpc_destroy(ldpc_ctx *ctx);
 
-/* Last error message of ctx (or of the last failed ldpc_create if ctx is
- * NULL).  The string lives until the next call on the same context. */
+/* Last error message of ctx (or, if ctx is NULL, of the last failed call
+ * that takes no context: ldpc_create, ldpc_decode_llrs_device_noalloc).  The
+ * string lives until the next call on the same context. */
 const char *ldpc_last_error(const ldpc_ctx *ctx);
 
 int ldpc_get_code_info(const ldpc_ctx *ctx, ldpc_code_info *info);
@@ -295,6 +296,26 @@ int ldpc_get_captures(ldpc_ctx *ctx, uint64_t *cw_ids, uint32_t cap, uint64_t *n
  * reference counterpart (the trained decoder's parameters). */
 int ldpc_set_dmin_params(ldpc_ctx *ctx, const float *alpha, const float *beta, uint32_t groups, uint32_t n_params,
                          int correction, float r_max);
+
+/* Allocation-free decode.  No counterpart in the reference: this entry point
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  It decodes
+ * like ldpc_decode_llrs_device_async, Profile T only (any AR4JA code and
+ * kernel variant whose kernels need no workspace; cfg as for ldpc_create),
+ * but needs no context: the kernels' scratch memory (work-queue counter, tally
+ * counters, iteration histogram) is one fixed 64 KiB device array per GPU
+ * inside libldpc, held until the process ends, so max_iter is limited to
+ * LDPC_NOALLOC_MAX_ITER.  The call allocates nothing and enqueues fill
+ * kernels and the decode on `stream` (a hipStream_t; NULL = the null stream)
+ * of `device`, then returns: the caller synchronizes.  Every call on one GPU
+ * shares that GPU's array, so calls that overlap on one GPU (concurrent
+ * streams or host threads) race and must be serialized by the caller.  The
+ * first call on a device resolves the array's address (hipGetSymbolAddress);
+ * make it outside graph capture.  Errors: ldpc_last_error(NULL). */
+#define LDPC_NOALLOC_MAX_ITER 8000
+int ldpc_decode_llrs_device_noalloc(const ldpc_code_desc *code, const ldpc_decoder_cfg *cfg, int device,
+                                    const void *d_llr, uint32_t n_cw, void *d_out_llr, void *d_out_iters,
+                                    void *d_out_ok, void *stream);
 
 /* Wait for all work on the context's stream. */
 int ldpc_sync(ldpc_ctx *ctx);

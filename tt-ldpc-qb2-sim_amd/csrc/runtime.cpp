@@ -350,6 +350,8 @@ void channel_params(const ldpc_code_info &info, double ebn0_db, float *mu, float
     *sigma = (float)std::sqrt(8.0 * rate * g);
 }
 
+void set_context_free_error(const std::string &msg) { g_create_error = msg; }
+
 }  // namespace ldpc
 
 extern "C" {

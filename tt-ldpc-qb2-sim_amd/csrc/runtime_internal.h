@@ -2,6 +2,8 @@
 #pragma once
 #include <stdint.h>
 
+#include <string>
+
 #include "../../include/ldpc_mi355x.h"
 #include "kernels.h"
 
@@ -17,5 +19,7 @@ const MultiVariant *profile_d_kernel(int k_id, int rate_id);
 void fill_code_info(int k_id, int rate_id, ldpc_code_info &info);
 // BPSK LLR mean / deviation at ebn0_db for rate k_info / n_tx.
 void channel_params(const ldpc_code_info &info, double ebn0_db, float *mu, float *sigma);
+// Message that ldpc_last_error(NULL) returns, for calls that take no context.
+void set_context_free_error(const std::string &msg);
 
 }  // namespace ldpc

@@ -89,6 +89,8 @@ SIGNATURES = {
     "ldpc_decode_llrs": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]),
     "ldpc_decode_llrs_device_async": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p,
                                                 C.c_void_p]),
+    "ldpc_decode_llrs_device_noalloc": (C.c_int, [C.POINTER(CodeDesc), C.POINTER(DecoderCfg), C.c_int, C.c_void_p,
+                                                  C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "ldpc_gen_channel": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, C.c_uint32, C.c_double, C.c_void_p]),
     "ldpc_channel_params": (C.c_int, [C.c_void_p, C.c_double, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     "ldpc_set_capture": (C.c_int, [C.c_void_p, C.c_uint32]),

@@ -169,3 +169,74 @@ def test_training_steps_run():
     print("loss", before, "->", after, [(l.step, l.loss) for l in log], flush=True)
     assert np.isfinite(after) and all(np.isfinite(l.loss) for l in log)
     assert float((model.alpha.detach() - 1).abs().max()) > 0 and float(model.beta.detach().abs().max()) > 0
+
+
+@pytest.mark.parametrize("k,rate,B,T,ckpt_every,ebn0", [
+    (1024, "4/5", 4, 3, 5, 2.6),       # degree-18 checks, checkpoints every 5 of 12 layer steps
+    (4096, "2/3", 3, 3, 7, 2.0),       # degree-10 checks, z = 256
+    (16384, "1/2", 2, 3, 5, 1.0),      # z = 2048, the large training configuration's code
+    (16384, "1/2", 2, 6, 60, 1.0),     # checkpoints every 5 iterations (60 layer steps), as trained at T = 50
+])
+def test_backward_other_codes_matches_torch_autograd(k, rate, B, T, ckpt_every, ebn0):
+    """The backward for degree 10 and 18 and for K=16384 (z = 2048), with the
+    min* correction and checkpoint intervals that are not multiples of the 12
+    layers per iteration, so that every recompute starts mid-iteration."""
+    code = codes.ar4ja(k, rate)
+    llr = llr_batch(code, B, ebn0, 41)
+    a, b = rand_params(code, 1, 42)
+    print(f"backward K={k} rate={rate} B={B} T={T} ckpt_every={ckpt_every}", flush=True)
+    L, gx, ga, gb = run_ext(code, llr, a, b, T, True, ckpt_every=ckpt_every)
+    x = torch.tensor(llr, device="cuda", requires_grad=True)
+    at = torch.tensor(a, device="cuda", requires_grad=True)
+    bt = torch.tensor(b, device="cuda", requires_grad=True)
+    Lr = REF.forward(code, x, at, bt, T, True, 0.0)
+    REF.loss(Lr, code.n_tx).backward()
+    np.testing.assert_array_equal(L.view(np.uint32), Lr.detach().cpu().numpy().view(np.uint32))
+    np.testing.assert_allclose(gx, x.grad.cpu().numpy(), rtol=1e-4, atol=1e-9)
+    np.testing.assert_allclose(ga, at.grad.cpu().numpy(), rtol=1e-4, atol=1e-8)
+    np.testing.assert_allclose(gb, bt.grad.cpu().numpy(), rtol=1e-4, atol=1e-8)
+    assert np.abs(ga).max() > 0 and np.abs(gb).max() > 0
+    # the checkpoint interval changes nothing
+    L1, gx1, ga1, gb1 = run_ext(code, llr, a, b, T, True, ckpt_every=1)
+    for p, q in ((L, L1), (gx, gx1), (ga, ga1), (gb, gb1)):
+        np.testing.assert_array_equal(p, q)
+
+
+def test_gradient_k16384_against_float64_finite_differences():
+    """K=16384 r1/2 with the shipped trained parameters (min* correction),
+    T = 12 at 2.0 dB, messages capped at 8, 4 codewords that all converge:
+    the GPU gradient against float64 central differences.
+
+    With 131,072 bits in the batch some message sits within 1e-3 of a kink
+    (a cap, a correction branch, a first/second minimum swap) for most
+    parameters, so the step-1e-3 quotient of the K=1024 test crosses kinks
+    here (measured: cosine 0.72).  The float64 oracle allows steps of 1e-6
+    and 1e-7; each parameter is compared with the quotient at the step that
+    lies closest, as a quotient across a kink is no derivative."""
+    import os
+    from ldpc_mi355x import dmin as D
+    code = codes.ar4ja(16384, "1/2")
+    T = 12
+    z = np.load(os.path.join(os.path.dirname(D.__file__), "data", "dmin_r12_k16384_t50_corr.npz"))
+    a, b = z["alpha"].astype(np.float32), z["beta"].astype(np.float32)
+    rng = np.random.default_rng(13)
+    mu, sg = channel_params(2.0, 0.5)
+    llr = np.zeros((4, code.n), np.float32)
+    llr[:, :code.n_tx] = (mu + sg * rng.standard_normal((4, code.n_tx))).astype(np.float32)
+    print("K=16384 finite differences, T=12, 4 codewords", flush=True)
+    L, _, ga, gb = run_ext(code, llr, a, b, T, True, ckpt_every=60, r_max=8.0)
+    assert ((L[:, :code.n_tx] <= 0).sum(1) == 0).all(), "every codeword of the batch converges"
+    idx = [(0, p) for p in (0, 7, 19, 30, 44, 59)]
+    an, fd = [], []
+    for which, g in (("alpha", ga), ("beta", gb)):
+        quot = [DO.fd_grad(code, llr, a, b, T, which, idx, eps, corr=True, r_max=8.0) for eps in (1e-6, 1e-7)]
+        for kk, (gg, p) in enumerate(idx):
+            an.append(float(g[gg, p]))
+            fd.append(min((float(q[kk]) for q in quot), key=lambda v: abs(v - an[-1])))
+            print(which, p, an[-1], [float(q[kk]) for q in quot], flush=True)
+    an, fd = np.array(an), np.array(fd)
+    err = np.abs(an - fd).max() / np.abs(fd).max()
+    cos = an @ fd / (np.linalg.norm(an) * np.linalg.norm(fd))
+    print("max error / max |fd|", err, "cos", cos, flush=True)
+    assert np.abs(fd).max() > 0
+    assert cos > 0.9999 and err < 1e-3

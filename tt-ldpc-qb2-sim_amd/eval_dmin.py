@@ -88,6 +88,10 @@ def main(argv=None):
     ap.add_argument("--forward", action="store_true",
                     help="decode the trained decoder with its training forward (T iterations, no early exit)")
     ap.add_argument("--forward-T", type=int, default=0, help="iterations of the --forward decode (0: the trained T)")
+    ap.add_argument("--segment", type=int, default=1 << 24,
+                    help="codewords per progress line (a point runs as consecutive segments of codeword ids)")
+    ap.add_argument("--decoders", nargs="+", choices=["trained", "approx_min_star"],
+                    default=["trained", "approx_min_star"], help="which decoders to run")
     args = ap.parse_args(argv)
 
     z = np.load(args.params, allow_pickle=False)
@@ -106,8 +110,10 @@ def main(argv=None):
                               "fer_ci95": [lo / done, hi / done], "ber": be / (done * k),
                               "gsps": done * n_tx / sec / 1e9}), flush=True)
         curves[f"trained_forward_T{t_fwd}"] = pts
-    runs = [(f"approx_min_star_{r}" if r != 200 else "approx_min_star", r, None) for r in args.ref_iters]
-    if not args.forward:
+    runs = []
+    if "approx_min_star" in args.decoders:
+        runs += [(f"approx_min_star_{r}" if r != 200 else "approx_min_star", r, None) for r in args.ref_iters]
+    if not args.forward and "trained" in args.decoders:
         runs += [(f"trained_T{t}", t, t) for t in args.iters]
     for name, max_iter, _ in runs:
         if name.startswith("approx_min_star"):
@@ -117,7 +123,14 @@ def main(argv=None):
             eng.set_dmin_params(z["alpha"], z["beta"], bool(z["correction"]), float(z["r_max"]))
         pts = []
         for eb in args.ebn0:
-            r = eng.run_point(eb, args.blocks, target_fe=args.target_fe, seed=args.seed, chunk_cw=1 << 20)
+            r = None
+            while r is None or (r.blocks < args.blocks and not (args.target_fe and r.frame_errors >= args.target_fe)):
+                n = min(args.segment, args.blocks - (r.blocks if r else 0))
+                fe_left = max(args.target_fe - (r.frame_errors if r else 0), 1) if args.target_fe else 0
+                r = eng.run_point(eb, n, target_fe=fe_left, seed=args.seed, first_cw=r.next_cw if r else 0,
+                                  chunk_cw=1 << 20, result=r)
+                print(f"# {name} {eb} dB: {r.blocks} codewords, {r.frame_errors} FE, {r.kernel_gsps:.1f} Gsps",
+                      file=sys.stderr, flush=True)
             lo, hi = poisson_ci95(r.frame_errors)
             pts.append((eb, r.fer))
             print(json.dumps({"decoder": name, "max_iter": max_iter, "ebn0_db": eb, "blocks": r.blocks,
@@ -126,7 +139,10 @@ def main(argv=None):
                               "mean_iterations": r.mean_iterations, "kernel_gsps": r.kernel_gsps}), flush=True)
         curves[name] = pts
         eng.close()
-    ref_name = [n for n in curves if n.startswith("approx_min_star")][-1]
+    refs = [n for n in curves if n.startswith("approx_min_star")]
+    if not refs:
+        return
+    ref_name = refs[-1]
     ref = curves[ref_name]
     summary = {"reference": ref_name}
     for name, pts in curves.items():

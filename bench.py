@@ -104,20 +104,44 @@ def poisson_ci95(k: int):
     return lo, hi
 
 
-def cpu_reference_baseline(seconds: float):
-    """Reference golden stepper, --sfpi --monte-carlo, all host threads."""
+def host_cpu():
+    """CPU model, logical CPUs this process may run on, and the physical cores behind them."""
+    cpus = sorted(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else list(range(os.cpu_count() or 1))
+    model, cores, cur = None, set(), {}
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f.read().splitlines() + [""]:
+                if not line.strip():
+                    if cur.get("processor") is not None and int(cur["processor"]) in cpus:
+                        cores.add((cur.get("physical id", "0"), cur.get("core id", cur["processor"])))
+                        model = model or cur.get("model name")
+                    cur = {}
+                    continue
+                k, _, v = line.partition(":")
+                cur[k.strip()] = v.strip()
+    except OSError:
+        pass
+    return {"cpu_model": model, "logical_cpus": len(cpus), "physical_cores": len(cores) or None}
+
+
+def cpu_threads() -> int:
+    return len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+
+
+def cpu_reference_baseline(seconds: float, ebn0: float = EBN0_DB):
+    """Reference golden stepper, --sfpi --monte-carlo, every CPU this process may use."""
     exe = os.path.join(ROOT, "oracle", "_ref", "stepper")
     if not os.path.exists(exe):
         return None
-    threads = os.cpu_count() or 1
+    threads = cpu_threads()
     # sized from a short probe so the sample takes about `seconds`
     probe = 400 * threads
     t0 = time.time()
-    p = subprocess.run([exe, "--sfpi", "--monte-carlo", str(EBN0_DB), str(probe), "1000000000", str(MAX_ITER),
+    p = subprocess.run([exe, "--sfpi", "--monte-carlo", str(ebn0), str(probe), "1000000000", str(MAX_ITER),
                         str(threads)], capture_output=True, text=True, timeout=600)
     dt = time.time() - t0
     blocks = max(int(probe * seconds / max(dt, 1e-3)) // 4 * 4, probe)
-    p = subprocess.run([exe, "--sfpi", "--monte-carlo", str(EBN0_DB), str(blocks), "1000000000", str(MAX_ITER),
+    p = subprocess.run([exe, "--sfpi", "--monte-carlo", str(ebn0), str(blocks), "1000000000", str(MAX_ITER),
                         str(threads)], capture_output=True, text=True, timeout=900)
     m = re.search(r"Host Throughput:\s*([0-9.]+) Msps", p.stdout)
     fe = re.search(r"Frame Errors:\s*(\d+)", p.stdout)
@@ -127,15 +151,15 @@ def cpu_reference_baseline(seconds: float):
         return None
     return {"value": float(m.group(1)) / 1e3, "unit": "Gsps", "cores": threads, "kind": "reference",
             "blocks": int(tb.group(1)) if tb else None, "frame_errors": int(fe.group(1)) if fe else None,
-            "beta": 1.0, "mean_iterations": float(it.group(1)) if it else None,
+            "beta": 1.0, "mean_iterations": float(it.group(1)) if it else None, "ebn0_db": ebn0,
             "sample": f"reference golden stepper (tests/test_circulant_dma_stepper.cpp, built from source) "
-                      f"--sfpi --monte-carlo {EBN0_DB} dB (its Monte Carlo mode decodes with beta 1), "
+                      f"--sfpi --monte-carlo {ebn0} dB (its Monte Carlo mode decodes with beta 1), "
                       f"{tb.group(1)} codewords, 200 max iter, "
                       f"{threads} OpenMP threads; mean iterations {it.group(1) if it else '?'}, "
                       f"FE {fe.group(1) if fe else '?'}"}
 
 
-def cpu_oracle_baseline(seconds: float):
+def cpu_oracle_baseline(seconds: float, beta: float = BETA):
     import numpy as np  # noqa: F401
 
     import oracle as O
@@ -143,18 +167,20 @@ def cpu_oracle_baseline(seconds: float):
     code = codes.ar4ja(1024, "1/2")
     g = 10 ** (EBN0_DB / 10)
     mu, sg = 4 * 0.5 * g, math.sqrt(8 * 0.5 * g)
-    threads = O.num_threads()
-    blocks, fe, t0, first = 0, 0, time.time(), 0
+    threads = cpu_threads()
+    blocks, fe, its, t0, first = 0, 0, 0, time.time(), 0
     while time.time() - t0 < seconds:
         n = 512 * threads
-        fe += O.run_point_t(code, 0xC0DE, first, n, mu, sg, MAX_ITER, BETA, threads)["frame_errors"]
+        r = O.run_point_t(code, 0xC0DE, first, n, mu, sg, MAX_ITER, beta, threads)
+        fe += r["frame_errors"]
+        its += r["iter_sum"]
         blocks += n
         first += n
     dt = time.time() - t0
     return {"value": blocks * N_TX / dt / 1e9, "unit": "Gsps", "cores": threads, "kind": "port",
-            "blocks": blocks, "frame_errors": fe,
-            "sample": f"CPU oracle (oracle/ldpc_oracle.c, Profile T), {blocks} codewords at {EBN0_DB} dB, "
-                      f"200 max iter, {threads} OpenMP threads"}
+            "blocks": blocks, "frame_errors": fe, "beta": beta, "mean_iterations": its / blocks,
+            "sample": f"CPU oracle (oracle/ldpc_oracle.c, Profile T, beta {beta}), {blocks} codewords at "
+                      f"{EBN0_DB} dB, 200 max iter, {threads} OpenMP threads"}
 
 
 def read_traffic():
@@ -215,7 +241,7 @@ def cpu_reference_scalar(seconds: float):
     exe = os.path.join(ROOT, "oracle", "_ref", "scalar_driver")
     if not os.path.exists(exe):
         return None
-    threads = os.cpu_count() or 1
+    threads = cpu_threads()
 
     def run(per_thread):
         p = subprocess.run([exe, "--monte-carlo", SCALAR_MATRIX, str(EBN0_DB), str(2560 - N_TX), str(MAX_ITER),
@@ -280,14 +306,16 @@ def bench_scalar(args) -> None:
         try:
             cpu = cpu_reference_scalar(args.cpu_seconds) or cpu_oracle_scalar(rows, n, args.cpu_seconds)
         except Exception as e:
-            cpu = {"value": None, "unit": "Gsps", "cores": os.cpu_count(), "kind": "port", "sample": f"failed: {e}"}
+            cpu = {"value": None, "unit": "Gsps", "cores": cpu_threads(), "kind": "port", "sample": f"failed: {e}"}
     fe_lo, fe_hi = poisson_ci95(acc.frame_errors)
     out = {
         "metric": "Gsps (channel sym/s) 1 GPU, AR4JA r1/2 K=1024 @2.0dB, Profile R (scalar_5w arithmetic)",
         "value": value, "unit": "Gsps", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "bf16",
-        "data": "synthetic: all-zero codeword, BPSK, AWGN from Philox4x32-10 + Box-Muller generated on device",
+        "data": ("synthetic: random info bits (Philox4x32-10), AR4JA-encoded on device, BPSK, AWGN from "
+                 "Philox4x32-10 + Box-Muller generated on device" if args.random_info else
+                 "synthetic: all-zero codeword, BPSK, AWGN from Philox4x32-10 + Box-Muller generated on device"),
         "config": {"workload": "Profile R on matrices/AR4JA_r45_4c_128c_r12.chinn.out (rows in file order, run "
                                "in 37 passes of <= 64), 512 punctured, Eb/N0 2.0 dB, 200 max iterations, 2^20 "
                                "codewords per step",
@@ -379,7 +407,7 @@ def bench_batch9(args) -> None:
         try:
             cpu = cpu_oracle_batch9(args.cpu_seconds)
         except Exception as e:
-            cpu = {"value": None, "unit": "Gsps", "cores": os.cpu_count(), "kind": "port", "sample": f"failed: {e}"}
+            cpu = {"value": None, "unit": "Gsps", "cores": cpu_threads(), "kind": "port", "sample": f"failed: {e}"}
     per_code = {f"K{k} r{r}": {"ebn0_db": eb, "fer": a.fer, "frame_errors": a.frame_errors, "ber": a.ber,
                                 "mean_iterations": a.mean_iterations, "n_tx": a.n_tx}
                 for (k, r), eb, a in zip(codes9, ebn0, acc)}
@@ -388,7 +416,9 @@ def bench_batch9(args) -> None:
         "value": syms / wall / 1e9, "unit": "Gsps", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "bf16",
-        "data": "synthetic: all-zero codeword, BPSK, AWGN from Philox4x32-10 + Box-Muller generated on device",
+        "data": ("synthetic: random info bits (Philox4x32-10), AR4JA-encoded on device, BPSK, AWGN from "
+                 "Philox4x32-10 + Box-Muller generated on device" if args.random_info else
+                 "synthetic: all-zero codeword, BPSK, AWGN from Philox4x32-10 + Box-Muller generated on device"),
         "config": {"workload": f"nine AR4JA codes (K 1024/4096/16384 x r1/2, 2/3, 4/5), {n} codewords of each "
                                f"per step, per-code Eb/N0 in [1, 3] dB, 200 max iterations, Profile T, one "
                                f"captured HIP graph per step", "codewords_per_code_per_step": n,
@@ -419,6 +449,9 @@ def main():
     ap.add_argument("--blocks", type=int, default=10 ** 9, help="codewords of the floor workload")
     ap.add_argument("--floor-first", type=int, default=0,
                     help="first codeword id of the floor workload (continue an earlier run of the same seed)")
+    ap.add_argument("--random-info", action="store_true",
+                    help="r12k1024: random information bits, encoded on the device, instead of the all-zero "
+                         "codeword (errors counted against the sent bits)")
     ap.add_argument("--floor-profile", choices=["T", "R"], default="T",
                     help="decoder of the floor workload: T (golden arithmetic; reference silicon 1.53e-9) or R "
                          "(scalar_5w arithmetic on the .chinn.out; reference 2.36e-9)")
@@ -442,6 +475,8 @@ def main():
         return
     from ldpc_mi355x import decoder
     eng = decoder.Engine(1024, "1/2", max_iter=MAX_ITER, beta=BETA, device=0)
+    if args.random_info:
+        eng.set_codeword_mode(True)
     seed = 0x5EED2026
     first = 0
     for _ in range(args.warmup):
@@ -469,9 +504,31 @@ def main():
         try:
             cpu = cpu_reference_baseline(args.cpu_seconds) or cpu_oracle_baseline(args.cpu_seconds)
         except Exception as e:  # a baseline failure must not hide the GPU number
-            cpu = {"value": None, "unit": "Gsps", "cores": os.cpu_count(), "kind": "reference",
+            cpu = {"value": None, "unit": "Gsps", "cores": cpu_threads(), "kind": "reference",
                    "sample": f"failed: {e}"}
     match = fer_match(acc.frame_errors, acc.blocks, cpu)
+    calib, match_15 = None, None
+    if cpu and cpu.get("kind") == "reference" and cpu.get("value"):
+        cpu.update(host_cpu())
+        try:
+            # calibration: this build's CPU oracle against the reference binary on the same
+            # cores, same workload (2.0 dB, beta 1 as the stepper's Monte Carlo mode)
+            port = cpu_oracle_baseline(args.cpu_seconds * 2 / 3, beta=1.0)
+            calib = {"oracle_gsps": port["value"], "reference_gsps": cpu["value"],
+                     "oracle_over_reference": port["value"] / cpu["value"], "cores": port["cores"],
+                     "oracle_blocks": port["blocks"], "oracle_mean_iterations": port["mean_iterations"]}
+            # a second FER match where frame errors are plentiful: 1.5 dB
+            ref15 = cpu_reference_baseline(args.cpu_seconds, 1.5)
+            eng15 = decoder.Engine(1024, "1/2", max_iter=MAX_ITER, beta=1.0, device=0)
+            r15 = eng15.run_point(1.5, 4 * CW_PER_STEP, seed=seed ^ 0x15, chunk_cw=CW_PER_STEP)
+            eng15.close()
+            match_15 = fer_match(r15.frame_errors, r15.blocks, ref15)
+            if match_15:
+                match_15.update({"ebn0_db": 1.5, "beta": 1.0, "gpu_mean_iterations": r15.mean_iterations,
+                                 "cpu_mean_iterations": ref15.get("mean_iterations"),
+                                 "cpu_gsps": ref15.get("value")})
+        except Exception as e:  # extra evidence only: never hide the headline number
+            calib = calib or {"failed": str(e)}
     if cpu and cpu.get("beta", BETA) != BETA:
         # The reference stepper's --monte-carlo mode decodes with beta = 1
         # (tests/test_circulant_dma_stepper.cpp:708, 1029): match its FER against a
@@ -500,17 +557,21 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "bf16",
-        "data": "synthetic: all-zero codeword, BPSK, AWGN from Philox4x32-10 + Box-Muller generated on device",
+        "data": ("synthetic: random info bits (Philox4x32-10), AR4JA-encoded on device, BPSK, AWGN from "
+                 "Philox4x32-10 + Box-Muller generated on device" if args.random_info else
+                 "synthetic: all-zero codeword, BPSK, AWGN from Philox4x32-10 + Box-Muller generated on device"),
         "config": {
             "workload": "AR4JA r1/2 K=1024 (N=2560, 2048 tx symbols), Eb/N0 2.0 dB, 200 max iterations, "
                         "Profile T (golden tree Approx-Min*, beta 15/16), 2^20 codewords per step",
             "code": "AR4JA r1/2 K=1024", "ebn0_db": EBN0_DB, "max_iter": MAX_ITER, "beta": BETA,
             "codewords_per_step": CW_PER_STEP, "symbols_per_codeword": N_TX,
+            "codeword": "random info bits, encoded on device" if args.random_info else "all-zero",
         },
         "roofline": {
             "bound": "valu", "achieved": ops_tops, "peak": PEAK_VALU_TOPS, "unit": "Tlane-op/s",
             "frac": ops_tops / PEAK_VALU_TOPS, "traffic": read_traffic(),
-            "kernel": "k_monte_carlo", "kernel_ms_per_launch": per_launch_s * 1e3,
+            "kernel": "k_monte_carlo_cw" if args.random_info else "k_monte_carlo",
+            "kernel_ms_per_launch": per_launch_s * 1e3,
             "algorithmic_ops_per_launch": OPS_PER_CWI * cwi_per_launch,
             "lds": {"achieved": lds_tbs, "peak": PEAK_LDS_TBS, "unit": "TB/s", "frac": lds_tbs / PEAK_LDS_TBS},
         },
@@ -520,6 +581,8 @@ def main():
         "kernel_gsps": acc.blocks * N_TX / kernel_s / 1e9,
         "speedup_vs_cpu": (value / cpu["value"]) if cpu and cpu.get("value") else None,
         "fer_match": match,
+        "fer_match_1p5db": match_15,
+        "cpu_calibration": calib,
         "reference_published_gsps": 5.2873,
     }
     print(json.dumps(out), flush=True)

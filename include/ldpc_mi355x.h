@@ -223,6 +223,20 @@ int ldpc_gen_channel(ldpc_ctx *ctx, uint64_t seed, uint64_t first_cw, uint32_t n
  * (ldpc_sim_scalar.cpp:352-355). */
 int ldpc_channel_params(const ldpc_ctx *ctx, double ebn0_db, float *mu, float *sigma);
 
+/* Systematic encoder of an AR4JA code (needs no GPU; no reference
+ * counterpart: every reference engine sends the all-zero codeword,
+ * engines/scalar_5w/ldpc_decoder_core.h:258-297).  A codeword is (u, p):
+ * the k_info information bits, then 12 parity blocks of z bits, the last 4
+ * of them the punctured variables.  With a block's bits read as polynomial
+ * coefficients (bit t <-> x^t), p_r = sum_i g_{r,i} u_i mod x^z + 1 over
+ * the info blocks u_i.  ldpc_get_generator writes the g_{r,i} as
+ * [12][k_info / z][z / 32] uint32 words (bit s of word s / 32 = coefficient
+ * of x^s; at most cap words) and returns the total count; ldpc_encode
+ * writes the n bits (one byte each, 0 or 1) of each of n_cw codewords from
+ * their k_info info bits (one byte each). */
+int ldpc_get_generator(const ldpc_code_desc *code, uint32_t *out, int32_t cap);
+int ldpc_encode(const ldpc_code_desc *code, const uint8_t *info, uint32_t n_cw, uint8_t *codeword);
+
 /* One variant's measurement by ldpc_autotune. */
 typedef struct ldpc_tune_result {
     uint32_t variant;
@@ -316,6 +330,33 @@ int ldpc_set_dmin_params(ldpc_ctx *ctx, const float *alpha, const float *beta, u
 int ldpc_decode_llrs_device_noalloc(const ldpc_code_desc *code, const ldpc_decoder_cfg *cfg, int device,
                                     const void *d_llr, uint32_t n_cw, void *d_out_llr, void *d_out_iters,
                                     void *d_out_ok, void *stream);
+
+/* Statistical check of the channel generator (no reference counterpart):
+ * counts[k] = number of the normals z behind the channel of codewords
+ * first_cw .. first_cw + n_cw - 1 (n_tx per codeword, the Philox4x32-10 +
+ * Box-Muller stream of ldpc_gen_channel) with |z| > thresholds[k],
+ * 1 <= n_thr <= 16, n_cw <= 2^40.  The stream's u = x 2^-32 + 2^-33 bounds
+ * |z| by sqrt(66 ln 2) = 6.764. */
+int ldpc_noise_tails(ldpc_ctx *ctx, uint64_t seed, uint64_t first_cw, uint64_t n_cw, const float *thresholds,
+                     uint32_t n_thr, uint64_t *counts);
+
+/* Codeword mode of ldpc_run_point and ldpc_gen_channel.  No reference
+ * counterpart: every reference engine sends the all-zero codeword
+ * (ldpc_decoder_core.h:258-297; stepper:744-757), which is the default here
+ * too.  LDPC_CODEWORD_RANDOM sends random information bits, systematically
+ * encoded (ldpc_encode): the info bits of codeword w are Philox4x32-10 with
+ * counter (w_lo, w_hi, j, 1) for j = 0 .. k_info/128 - 1 and key seed, 128
+ * bits per call (bit b of output word d = info bit 128 j + 32 d + b); BPSK
+ * sends bit 1 as -mu on the same noise as the all-zero mode.  The tally
+ * then counts against the sent bits: a frame error is a codeword whose
+ * syndrome never cleared or cleared with an info-bit hard decision
+ * (bf16 < 0 -> 1) other than the sent bit; its bit errors are the info bits
+ * not strictly on the sent side (bf16 +-0 or the wrong sign), which for an
+ * all-zero word is the golden's bf16 <= 0.0f.  hist[0] still counts only the
+ * codewords that never converged.  Built for the Profile T kernel of AR4JA
+ * r1/2 K=1024 (variant 0); other contexts return LDPC_ERR_UNSUPPORTED. */
+enum { LDPC_CODEWORD_ZERO = 0, LDPC_CODEWORD_RANDOM = 1 };
+int ldpc_set_codeword_mode(ldpc_ctx *ctx, uint32_t mode);
 
 /* Wait for all work on the context's stream. */
 int ldpc_sync(ldpc_ctx *ctx);

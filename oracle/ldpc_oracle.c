@@ -567,6 +567,101 @@ EXPORT void orc_gen_channel(uint64_t seed, uint64_t first_cw, int n_cw, float mu
 }
 
 /* ------------------------------------------------------------------------- */
+/* Random-info-bit codewords.  No reference counterpart: every reference     */
+/* engine sends the all-zero codeword (ldpc_decoder_core.h:258-297;          */
+/* stepper:744-757).  Info bits of codeword w: Philox4x32-10 with counter    */
+/* (w_lo, w_hi, j, 1), j = 0 .. k_info/128 - 1, 128 bits per call, bit b of */
+/* output word d = info bit 128 j + 32 d + b.  Parity bits: a dense          */
+/* generator P of k_info rows, each row the n - k_info parity bits packed 64 */
+/* per word (pw words): parity = XOR of the rows of the set info bits.  The  */
+/* codeword is (u, parity); BPSK sends bit c as +mu (c = 0) or -mu (c = 1):  */
+/* LLR = RNE(+-mu + sigma z) on the all-zero mode's noise z; punctured = +0. */
+/* ------------------------------------------------------------------------- */
+static void info_bits(const uint32_t key[2], uint64_t w, int k_info, uint8_t *c) {
+    for (int j = 0; j < k_info / 128; ++j) {
+        uint32_t ctr[4] = {(uint32_t)w, (uint32_t)(w >> 32), (uint32_t)j, 1u}, x[4];
+        orc_philox4x32_10(ctr, key, x);
+        for (int b = 0; b < 128; ++b) c[128 * j + b] = (uint8_t)((x[b / 32] >> (b % 32)) & 1u);
+    }
+}
+
+static void encode_dense(const uint64_t *P, int pw, int k_info, int n, uint8_t *c) {
+    uint64_t acc[1024];
+    for (int k = 0; k < pw; ++k) acc[k] = 0;
+    for (int k = 0; k < k_info; ++k)
+        if (c[k])
+            for (int q = 0; q < pw; ++q) acc[q] ^= P[(size_t)k * pw + q];
+    for (int v = k_info; v < n; ++v) c[v] = (uint8_t)((acc[(v - k_info) / 64] >> ((v - k_info) % 64)) & 1u);
+}
+
+static void gen_one_cw(const uint32_t key[2], uint64_t w, float mu, float sigma, int n_tx, int n,
+                       const uint8_t *c, uint16_t *L) {
+    for (int blk = 0; blk < (n_tx + 3) / 4; ++blk) {
+        uint32_t ctr[4] = {(uint32_t)w, (uint32_t)(w >> 32), (uint32_t)blk, 0u}, x[4];
+        orc_philox4x32_10(ctr, key, x);
+        float z[4];
+        orc_box_muller(x[0], x[1], &z[0], &z[1]);
+        orc_box_muller(x[2], x[3], &z[2], &z[3]);
+        for (int j = 0; j < 4 && 4 * blk + j < n_tx; ++j) {
+            float t = sigma * z[j];
+            float l = (c[4 * blk + j] ? -mu : mu) + t;
+            L[4 * blk + j] = f2bf(l);
+        }
+    }
+    for (int v = n_tx; v < n; ++v) L[v] = 0;
+}
+
+/* Tally of a random codeword c: the golden's rule (stepper:799-815) with the
+ * errors counted against the sent bits.  A codeword is a frame error if its
+ * syndrome never cleared, or if it cleared on a word whose info-bit hard
+ * decisions (bf16 < 0 -> 1, stepper:781) differ from the sent bits.  A frame
+ * error adds the info bits k < k_info that are not strictly on the sent side
+ * (bf16 == +-0, or the sign differs from c_k): for c = 0 that is the
+ * golden's bf16 <= 0.0f.  hist / iterations as tally_t (hist[0] counts the
+ * codewords that never converged).                                          */
+static void tally_cw(const uint16_t *L, const uint8_t *c, int it, int k_info, int max_iter, uint64_t *fe, uint64_t *be,
+                     uint64_t *its, uint64_t *h) {
+    int mism = 0;
+    for (int k = 0; k < k_info; ++k) mism |= hard_neg(L[k]) != c[k];
+    if (it) {
+        *its += (uint64_t)it;
+        h[it] += 1;
+    } else {
+        *its += (uint64_t)max_iter;
+        h[0] += 1;
+    }
+    if (!it || mism) {
+        *fe += 1;
+        for (int k = 0; k < k_info; ++k) *be += (uint64_t)(((L[k] & 0x7FFFu) == 0) || ((L[k] >> 15) != c[k]));
+    }
+}
+
+/* Channel LLRs (and, if bits != NULL, the codeword bits) of random codewords
+ * first_cw .. first_cw + n_cw - 1. */
+EXPORT int orc_gen_channel_cw(uint64_t seed, uint64_t first_cw, int n_cw, float mu, float sigma, int n_tx, int n,
+                              int k_info, const uint64_t *P, int pw, uint16_t *out, uint8_t *bits) {
+    if (k_info % 128 || pw > 1024 || 64 * pw < n - k_info) return -1;
+    const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+#ifdef _OPENMP
+#pragma omp parallel
+#endif
+    {
+        uint8_t *c = (uint8_t *)malloc((size_t)n);
+#ifdef _OPENMP
+#pragma omp for schedule(static)
+#endif
+        for (int i = 0; i < n_cw; ++i) {
+            info_bits(key, first_cw + (uint64_t)i, k_info, c);
+            encode_dense(P, pw, k_info, n, c);
+            gen_one_cw(key, first_cw + (uint64_t)i, mu, sigma, n_tx, n, c, out + (size_t)i * n);
+            if (bits) memcpy(bits + (size_t)i * n, c, (size_t)n);
+        }
+        free(c);
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------------- */
 /* Monte Carlo point, Profile T (stepper:708-856)                            */
 /* counters: [0] blocks, [1] frame errors, [2] bit errors, [3] iteration sum */
 /* hist[i], i = 1..max_iter: codewords that converged at iteration i;        */
@@ -655,6 +750,46 @@ EXPORT int orc_run_point_t(int n, int m, const int32_t *rp, const int32_t *col, 
 #endif
         for (int i = 0; i <= max_iter; ++i) hist[i] += h[i];
         free(L); free(R); free(h);
+    }
+    counters[0] = blocks; counters[1] = fe; counters[2] = be; counters[3] = its;
+    return 0;
+}
+
+/* Monte Carlo point, Profile T, random codewords (tally_cw's rule). */
+EXPORT int orc_run_point_t_cw(int n, int m, const int32_t *rp, const int32_t *col, const int32_t *sched, int n_tx,
+                              int k_info, const uint64_t *P, int pw, uint64_t seed, uint64_t first_cw, uint64_t n_cw,
+                              float mu, float sigma, int max_iter, float beta, int threads, uint64_t *counters,
+                              uint64_t *hist) {
+    if (check_degrees_t(m, rp) || k_info % 128 || pw > 1024 || 64 * pw < n - k_info) return -1;
+    uint64_t blocks = 0, fe = 0, be = 0, its = 0;
+    const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    memset(hist, 0, sizeof(uint64_t) * (size_t)(max_iter + 1));
+#ifdef _OPENMP
+    if (threads <= 0) threads = omp_get_max_threads();
+#pragma omp parallel num_threads(threads) reduction(+ : blocks, fe, be, its)
+#endif
+    {
+        uint16_t *L = (uint16_t *)malloc(sizeof(uint16_t) * (size_t)n);
+        uint16_t *R = (uint16_t *)malloc(sizeof(uint16_t) * (size_t)rp[m]);
+        uint8_t *c = (uint8_t *)malloc((size_t)n);
+        uint64_t *h = (uint64_t *)calloc((size_t)max_iter + 1, sizeof(uint64_t));
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 16)
+#endif
+        for (int64_t i = 0; i < (int64_t)n_cw; ++i) {
+            const uint64_t w = first_cw + (uint64_t)i;
+            info_bits(key, w, k_info, c);
+            encode_dense(P, pw, k_info, n, c);
+            gen_one_cw(key, w, mu, sigma, n_tx, n, c, L);
+            blocks += 1;
+            tally_cw(L, c, decode_one_t(L, R, m, rp, col, sched, max_iter, beta), k_info, max_iter, &fe, &be, &its,
+                     h);
+        }
+#ifdef _OPENMP
+#pragma omp critical
+#endif
+        for (int i = 0; i <= max_iter; ++i) hist[i] += h[i];
+        free(L); free(R); free(c); free(h);
     }
     counters[0] = blocks; counters[1] = fe; counters[2] = be; counters[3] = its;
     return 0;

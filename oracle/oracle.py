@@ -67,6 +67,13 @@ def lib():
         L.orc_run_llrs_t.restype = C.c_int
         L.orc_run_llrs_t.argtypes = [C.c_int, C.c_int, i32p, i32p, i32p, C.c_int, C.c_int, u16p, C.c_int, C.c_float,
                                      C.c_int, u64p, u64p]
+        L.orc_gen_channel_cw.restype = C.c_int
+        L.orc_gen_channel_cw.argtypes = [C.c_uint64, C.c_uint64, C.c_int, C.c_float, C.c_float, C.c_int, C.c_int,
+                                         C.c_int, u64p, C.c_int, u16p, C.c_void_p]
+        L.orc_run_point_t_cw.restype = C.c_int
+        L.orc_run_point_t_cw.argtypes = [C.c_int, C.c_int, i32p, i32p, i32p, C.c_int, C.c_int, u64p, C.c_int,
+                                         C.c_uint64, C.c_uint64, C.c_uint64, C.c_float, C.c_float, C.c_int, C.c_float,
+                                         C.c_int, u64p, u64p]
         L.orc_decode_r.restype = C.c_int
         L.orc_decode_r.argtypes = [C.c_int, C.c_int, i32p, i32p, C.c_int, u16p, C.c_int, C.c_int, C.c_float, C.c_float,
                                    u16p, u32p, u8p, C.c_int]
@@ -214,6 +221,83 @@ def run_llrs_t(code, llr: np.ndarray, max_iter: int = 200, beta: float = 0.9375,
                               cnt, hist)
     if rc:
         raise ValueError("oracle Profile T supports check degree 3 and even degrees 4..18")
+    return dict(blocks=int(cnt[0]), frame_errors=int(cnt[1]), bit_errors=int(cnt[2]), iter_sum=int(cnt[3]),
+                hist=hist)
+
+
+# --- random-info-bit codewords (no reference counterpart) ------------------
+
+_GEN_CACHE = {}
+
+
+def dense_parity_generator(code) -> np.ndarray:
+    """Dense systematic generator of a code whose last m variables carry the
+    parity: P (k x m over GF(2)) with parity = u P, from Gauss-Jordan
+    elimination of [H_p | H_u] on bit-packed rows (H_p the last m columns,
+    which must be invertible).  Plain dense linear algebra, independent of
+    the product's quasi-cyclic encoder.  Returns P packed: k rows of
+    ceil(m / 64) little-endian uint64 words."""
+    key = (code.name, code.n, code.m)
+    if key in _GEN_CACHE:
+        return _GEN_CACHE[key]
+    k, m = code.k, code.m
+    assert code.n - k == m, "the parity part must be square"
+    H = code.dense()
+    A = np.concatenate([H[:, k:], H[:, :k]], axis=1).astype(np.uint8)   # [H_p | H_u]
+    W = (A.shape[1] + 63) // 64
+    bits = np.zeros((m, W * 64), dtype=np.uint8)
+    bits[:, :A.shape[1]] = A
+    rows = np.packbits(bits.reshape(m, W, 64)[:, :, ::-1], axis=2).view(">u8").reshape(m, W).astype(np.uint64)
+
+    def col_bits(c):
+        return (rows[:, c // 64] >> np.uint64(c % 64)) & np.uint64(1)
+
+    for c in range(m):
+        piv = c + int(np.argmax(col_bits(c)[c:]))
+        if not col_bits(c)[piv]:
+            raise ValueError("parity part of H is singular")
+        if piv != c:
+            rows[[c, piv]] = rows[[piv, c]]
+        hit = col_bits(c).astype(bool)
+        hit[c] = False
+        rows[hit] ^= rows[c]
+    # rows now hold [I | H_p^-1 H_u]; P = (H_p^-1 H_u)^T
+    right = np.zeros((m, k), dtype=np.uint8)
+    for c in range(k):
+        right[:, c] = col_bits(m + c).astype(np.uint8)
+    Pt = right.T                                                         # k x m
+    pw = (m + 63) // 64
+    pad = np.zeros((k, pw * 64), dtype=np.uint8)
+    pad[:, :m] = Pt
+    P = np.packbits(pad.reshape(k, pw, 64)[:, :, ::-1], axis=2).view(">u8").reshape(k, pw).astype(np.uint64)
+    P = np.ascontiguousarray(P)
+    _GEN_CACHE[key] = P
+    return P
+
+
+def gen_channel_cw(code, seed: int, first_cw: int, n_cw: int, mu: float, sigma: float, with_bits: bool = False):
+    """Channel LLRs [n_cw, n] of random-info-bit codewords (and their bits)."""
+    P = dense_parity_generator(code)
+    out = np.empty((n_cw, code.n), dtype=np.uint16)
+    bits = np.empty((n_cw, code.n), dtype=np.uint8) if with_bits else None
+    rc = lib().orc_gen_channel_cw(seed, first_cw, n_cw, mu, sigma, code.n_tx, code.n, code.k, P, P.shape[1], out,
+                                  bits.ctypes.data if with_bits else None)
+    if rc:
+        raise ValueError("unsupported code for the random-codeword oracle")
+    return (out, bits) if with_bits else out
+
+
+def run_point_t_cw(code, seed: int, first_cw: int, n_cw: int, mu: float, sigma: float, max_iter: int = 200,
+                   beta: float = 0.9375, threads: int = 0):
+    """Monte Carlo of random-info-bit codewords: dict(blocks, frame_errors, bit_errors, iter_sum, hist)."""
+    P = dense_parity_generator(code)
+    rp, col, sched = code_arrays(code)
+    cnt = np.zeros(4, dtype=np.uint64)
+    hist = np.zeros(max_iter + 1, dtype=np.uint64)
+    rc = lib().orc_run_point_t_cw(code.n, code.m, rp, col, sched, code.n_tx, code.k, P, P.shape[1], seed, first_cw,
+                                  n_cw, mu, sigma, max_iter, beta, threads, cnt, hist)
+    if rc:
+        raise ValueError("unsupported code for the random-codeword oracle")
     return dict(blocks=int(cnt[0]), frame_errors=int(cnt[1]), bit_errors=int(cnt[2]), iter_sum=int(cnt[3]),
                 hist=hist)
 

@@ -166,3 +166,23 @@ def test_circulant_shim_writes_engine_csv(tmp_path, oracle, code12):
     mu, sg = channel_params(1.5)
     o = oracle.run_point_t(code12, 9, 0, 3520, mu, sg, 200, 0.9375)
     assert rows[-1].split(",")[2:4] == [str(o["bit_errors"]), str(o["frame_errors"])]
+
+
+@pytest.mark.gpu
+def test_circulant_engine_random_info_csv(tmp_path, monkeypatch, oracle, code12):
+    monkeypatch.chdir(tmp_path)
+    print("[gpu-case] run_simulation --engine circulant --random_info", flush=True)
+    out = RS.main(["--chinn", MATRIX, "--ebn0_start", "1.5", "--ebn0_end", "1.5", "--punctured", "512",
+                   "--engine", "circulant", "--random_info", "--max_blocks", "4400", "--batch_size", "5",
+                   "--max_iter", "200", "--min_errors", "100000", "--output", "r.csv"])
+    row = open(out).read().splitlines()[1].split(",")
+    mu, sg = channel_params(1.5)
+    o = oracle.run_point_t_cw(code12, 133742, 0, 4400, mu, sg, 200, 0.9375)
+    assert row[:4] == ["1.50", "4400", str(o["bit_errors"]), str(o["frame_errors"])]
+
+
+def test_random_info_needs_the_inprocess_circulant_engine(tmp_path, monkeypatch):
+    monkeypatch.chdir(tmp_path)
+    with pytest.raises(SystemExit):
+        RS.main(["--chinn", MATRIX, "--ebn0_start", "1.5", "--ebn0_end", "1.5", "--random_info", "--binary",
+                 "/bin/false"])

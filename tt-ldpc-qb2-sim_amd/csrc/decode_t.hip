@@ -223,11 +223,102 @@ __device__ __forceinline__ uint32_t wave_info_errors(const uint16_t *Ls, int lan
     return be;
 }
 
+// Random-info-bit codewords (no reference counterpart: the reference sends
+// the all-zero codeword, ldpc_decoder_core.h:258-297, stepper:744-757).  The
+// info bits of codeword w are Philox(ctr = (w_lo, w_hi, j, 1), key = seed),
+// j = 0..7, 128 bits per call (bit b of word d = info bit 128 j + 32 d + b).
+// Parity: the code is quasi-cyclic with 128-bit blocks; transmitted parity
+// block r (variables 1024 + 128 r + t) is p_r = sum_i g_{r,i} u_i in
+// GF(2)[x] / (x^128 + 1), i.e. bit t = parity(g_{r,i} AND w_{i,t}) summed over
+// the info blocks i, with w_{i,t}[s] = u_i[(t - s) mod 128].  g comes from the
+// host's encoder: 8 x 8 x 4 words in device memory ([transmitted parity block
+// r][info block i][128 bits]), read at the point of use.
+constexpr int CW_GEN_WORDS = 8 * 8 * 4;
+
+// Info bits of codeword w into ub[0..31]; the doubled bit-reversed blocks
+// v_i[k] = u_i[(-k) mod 128] into scr[8 i .. 8 i + 7] (scr: LDS scratch).
+__device__ __forceinline__ void wave_info_bits(uint32_t *ub, uint32_t *scr, int lane, uint32_t k0, uint32_t k1,
+                                               uint64_t w) {
+    if (lane < KINFO / 128) {
+        const u32x4 x = philox4x32_10((uint32_t)w, (uint32_t)(w >> 32), (uint32_t)lane, 1u, k0, k1);
+        *reinterpret_cast<uint4 *>(ub + 4 * lane) = make_uint4(x.x, x.y, x.z, x.w);
+        // bit-reverse the 128 bits (b[k] = u[127 - k]), then rotate left by one (v[k] = b[k - 1])
+        const uint32_t b0 = __brev(x.w), b1 = __brev(x.z), b2 = __brev(x.y), b3 = __brev(x.x);
+        const uint4 v = make_uint4((b0 << 1) | (b3 >> 31), (b1 << 1) | (b0 >> 31), (b2 << 1) | (b1 >> 31),
+                                   (b3 << 1) | (b2 >> 31));
+        *reinterpret_cast<uint4 *>(scr + 8 * lane) = v;
+        *reinterpret_cast<uint4 *>(scr + 8 * lane + 4) = v;
+    }
+}
+
+// w_{i,t} = bits [o, o + 128) of the doubled v_i, o = (128 - t) mod 128.
+__device__ __forceinline__ void info_window(const uint32_t *vv, int t, uint32_t (&w)[4]) {
+    const int o = (Z - t) & (Z - 1), d = o >> 5;
+    const uint32_t sh = (uint32_t)(o & 31);
+    uint32_t x[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) x[k] = vv[d + k];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) w[k] = __builtin_amdgcn_alignbit(x[k + 1], x[k], sh);
+}
+
+// Transmitted parity bits of the codeword into scr[64 .. 95] (bit p of the
+// 1024 = variable 1024 + p).  Lane l computes positions l and l + 64 of
+// every parity block.
+struct Words4 {
+    uint32_t x, y, z, w;
+};
+typedef __attribute__((address_space(4))) const Words4 const_words4;
+
+__device__ __forceinline__ void wave_encode(uint32_t *scr, int lane, const uint32_t *__restrict__ gen) {
+    // The table is read through the constant address space (scalar loads:
+    // uniform, no VGPRs), from an opaque copy of its pointer so that the
+    // loads stay inside the codeword loop, and one info block at a time
+    // (scheduling barriers): kept live across the decode, or loaded all at
+    // once, its 256 words spill registers and cut the occupancy.
+    asm volatile("" : "+s"(gen));
+    const_words4 *g4 = (const_words4 *)(const void *)gen;
+    uint32_t acc[8][2];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) acc[r][0] = acc[r][1] = 0u;
+#pragma unroll
+    for (int i = 0; i < KINFO / 128; ++i) {
+        __builtin_amdgcn_sched_barrier(0);
+        uint32_t wa[4], wb[4];
+        info_window(scr + 8 * i, lane, wa);
+        info_window(scr + 8 * i, lane + 64, wb);
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            const uint32_t gx = g4[r * 8 + i].x, gy = g4[r * 8 + i].y, gz = g4[r * 8 + i].z, gw = g4[r * 8 + i].w;
+            acc[r][0] ^= (gx & wa[0]) ^ (gy & wa[1]) ^ (gz & wa[2]) ^ (gw & wa[3]);
+            acc[r][1] ^= (gx & wb[0]) ^ (gy & wb[1]) ^ (gz & wb[2]) ^ (gw & wb[3]);
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < 8; ++r)
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const uint64_t m = __ballot(__popc(acc[r][h]) & 1u);
+            if (lane == 0) *reinterpret_cast<uint2 *>(scr + 64 + 4 * r + 2 * h) = make_uint2((uint32_t)m, (uint32_t)(m >> 32));
+        }
+}
+
 // Channel LLRs of codeword w into Ls: Philox(ctr = (w_lo, w_hi, blk, 0),
-// key = seed) -> 4 normals -> RNE_bf16(mu + sigma z); punctured = +0.
-__device__ __forceinline__ void wave_gen_channel(uint16_t *Ls, int lane, uint64_t seed, uint64_t w, float mu,
-                                                 float sigma) {
+// key = seed) -> 4 normals -> RNE_bf16(+-mu + sigma z); punctured = +0.  The
+// all-zero codeword sends +mu everywhere.  A random codeword (RANDOM) sends
+// -mu where its bit is 1; its info bits stay in ub for the tally, and the
+// punctured tail of Ls serves as scratch for the encoder until it is zeroed.
+template <bool RANDOM>
+__device__ __forceinline__ void wave_gen_channel(uint16_t *Ls, uint32_t *ub, int lane, uint64_t seed, uint64_t w,
+                                                 float mu, float sigma, const uint32_t *__restrict__ gen) {
     const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
+    uint32_t *scr = reinterpret_cast<uint32_t *>(Ls + NTX);
+    if constexpr (RANDOM) {
+        wave_info_bits(ub, scr, lane, k0, k1, w);
+        wave_lds_fence();
+        wave_encode(scr, lane, gen);
+        wave_lds_fence();
+    }
 #pragma unroll 2
     for (int t = 0; t < NTX / 4 / 64; ++t) {
         const uint32_t blk = (uint32_t)(t * 64 + lane);
@@ -235,12 +326,41 @@ __device__ __forceinline__ void wave_gen_channel(uint16_t *Ls, int lane, uint64_
         float z0, z1, z2, z3;
         box_muller(x.x, x.y, z0, z1);
         box_muller(x.z, x.w, z2, z3);
-        uint32_t lo = (llr_bits(mu, sigma, z1) & 0xFFFF0000u) | (llr_bits(mu, sigma, z0) >> 16);
-        uint32_t hi = (llr_bits(mu, sigma, z3) & 0xFFFF0000u) | (llr_bits(mu, sigma, z2) >> 16);
+        float m0 = mu, m1 = mu, m2 = mu, m3 = mu;
+        if constexpr (RANDOM) {
+            const uint32_t v = 4 * blk;  // four consecutive variables, one nibble of the codeword bits
+            const uint32_t cw = (v < KINFO ? ub[v >> 5] : scr[64 + ((v - KINFO) >> 5)]) >> (v & 31);
+            m0 = (cw & 1u) ? -mu : mu;
+            m1 = (cw & 2u) ? -mu : mu;
+            m2 = (cw & 4u) ? -mu : mu;
+            m3 = (cw & 8u) ? -mu : mu;
+        }
+        uint32_t lo = (llr_bits(m1, sigma, z1) & 0xFFFF0000u) | (llr_bits(m0, sigma, z0) >> 16);
+        uint32_t hi = (llr_bits(m3, sigma, z3) & 0xFFFF0000u) | (llr_bits(m2, sigma, z2) >> 16);
         *reinterpret_cast<uint2 *>(Ls + 4 * blk) = make_uint2(lo, hi);
     }
+    if constexpr (RANDOM) wave_lds_fence();
     // punctured variables NTX..NV-1: 512 x bf16 +0 = 64 lanes x 16 B
     *reinterpret_cast<uint4 *>(Ls + NTX + 8 * lane) = make_uint4(0u, 0u, 0u, 0u);
+}
+
+// Tally of a random codeword (info bits in ub): mism = some info-bit hard
+// decision (bf16 < 0 -> 1, stepper:781) differs from the sent bit; errs = info
+// bits not strictly on the sent side (bf16 +-0, or the sign differs), which
+// for a sent 0 is the golden's bf16 <= 0.0f (stepper:801-809).
+__device__ __forceinline__ void wave_cw_errors(const uint16_t *Ls, const uint32_t *ub, int lane, bool &mism,
+                                               uint32_t &errs) {
+    uint64_t any = 0;
+    errs = 0;
+#pragma unroll
+    for (int t = 0; t < KINFO / 64; ++t) {
+        const int k = t * 64 + lane;
+        const uint16_t h = Ls[k];
+        const uint32_t c = (ub[k >> 5] >> (k & 31)) & 1u;
+        any |= __ballot((uint32_t)hard_neg(h) != c);
+        errs += (uint32_t)__popcll(__ballot((h & 0x7FFFu) == 0 || (uint32_t)(h >> 15) != c));
+    }
+    mism = any != 0;
 }
 
 // Work split of a launch: codewords [0, n_static) are dealt round-robin to the
@@ -309,37 +429,47 @@ __global__ void __launch_bounds__(64 * WAVES_PER_WG)
 // 64-byte HBM write each.
 constexpr int HIST_LDS = 256;
 
-__global__ void __launch_bounds__(64 * WAVES_PER_WG)
-    k_monte_carlo(uint64_t seed, uint64_t first_cw, uint32_t n_cw, float mu, float sigma, int max_iter, float beta6,
-                  float beta3, uint32_t *queue, unsigned long long *counters, unsigned long long *hist, Capture cap) {
-    __shared__ __attribute__((aligned(16))) uint16_t smem[WAVES_PER_WG][NV];
+template <bool RANDOM>
+__device__ __forceinline__ void monte_carlo(uint64_t seed, uint64_t first_cw, uint32_t n_cw, float mu, float sigma,
+                                            int max_iter, float beta6, float beta3, uint32_t *queue,
+                                            unsigned long long *counters, unsigned long long *hist, const Capture &cap,
+                                            const uint32_t *__restrict__ gen) {
+    // random codewords keep their info bits right after the wave's LLRs, so
+    // that one base address serves both (a second one costs registers)
+    __shared__ __attribute__((aligned(16))) uint16_t smem[WAVES_PER_WG][NV + (RANDOM ? 2 * KINFO / 32 : 0)];
     __shared__ uint32_t hist_s[HIST_LDS];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     for (int i = threadIdx.x; i < HIST_LDS; i += blockDim.x) hist_s[i] = 0u;
     __syncthreads();
     uint16_t *Ls = smem[wv];
+    uint32_t *ub = reinterpret_cast<uint32_t *>(Ls + NV);
     uint64_t blocks = 0, fe = 0, be = 0, its = 0;
     WorkSplit ws = work_split(n_cw);
     for (;;) {
         const uint32_t cw = next_codeword(ws, queue, lane);
         if (cw >= n_cw) break;
-        wave_gen_channel(Ls, lane, seed, first_cw + cw, mu, sigma);
+        wave_gen_channel<RANDOM>(Ls, ub, lane, seed, first_cw + cw, mu, sigma, gen);
         wave_lds_fence();
         const int it = decode_wave(Ls, lane, max_iter, beta6, beta3);
         blocks += 1;
-        if (it) {
-            its += (uint64_t)it;
+        its += it ? (uint64_t)it : (uint64_t)max_iter;
+        bool failed = !it;
+        if constexpr (RANDOM) {
+            bool mism;
+            uint32_t errs;
+            wave_cw_errors(Ls, ub, lane, mism, errs);
+            failed = failed || mism;
+            if (failed) be += errs;
         } else {
-            its += (uint64_t)max_iter;
-            fe += 1;
-            be += wave_info_errors(Ls, lane);
+            if (failed) be += wave_info_errors(Ls, lane);
         }
+        if (failed) fe += 1;
         if (lane == 0) {
             if (it < HIST_LDS)
                 atomicAdd(hist_s + it, 1u);
             else
                 atomicAdd(hist + it, 1ull);
-            if (!it) capture_failure(cap, first_cw + cw);
+            if (failed) capture_failure(cap, first_cw + cw);
         }
         wave_lds_fence();
     }
@@ -353,6 +483,41 @@ __global__ void __launch_bounds__(64 * WAVES_PER_WG)
     const int nb = max_iter + 1 < HIST_LDS ? max_iter + 1 : HIST_LDS;
     for (int i = threadIdx.x; i < nb; i += blockDim.x)
         if (hist_s[i]) atomicAdd(hist + i, (unsigned long long)hist_s[i]);
+}
+
+__global__ void __launch_bounds__(64 * WAVES_PER_WG)
+    k_monte_carlo(uint64_t seed, uint64_t first_cw, uint32_t n_cw, float mu, float sigma, int max_iter, float beta6,
+                  float beta3, uint32_t *queue, unsigned long long *counters, unsigned long long *hist, Capture cap) {
+    monte_carlo<false>(seed, first_cw, n_cw, mu, sigma, max_iter, beta6, beta3, queue, counters, hist, cap, nullptr);
+}
+
+// Random codewords.  The encoder and the tally against the sent bits push
+// the register count past 128, and the compiler then schedules the decode
+// for 2 waves per SIMD (214 VGPRs, measured 14% slower at 1.5 dB); pinned to
+// 4 waves per SIMD the decode loop keeps its registers and only the
+// once-per-codeword generation and tally spill (about 35 dwords per lane).
+__global__ void __launch_bounds__(64 * WAVES_PER_WG) __attribute__((amdgpu_waves_per_eu(4)))
+    k_monte_carlo_cw(uint64_t seed, uint64_t first_cw, uint32_t n_cw, float mu, float sigma, int max_iter,
+                     float beta6, float beta3, uint32_t *queue, unsigned long long *counters,
+                     unsigned long long *hist, Capture cap, const uint32_t *__restrict__ gen) {
+    monte_carlo<true>(seed, first_cw, n_cw, mu, sigma, max_iter, beta6, beta3, queue, counters, hist, cap, gen);
+}
+
+// Channel LLRs of random codewords, one wave per codeword, through the Monte
+// Carlo kernel's own generator.
+__global__ void __launch_bounds__(64 * WAVES_PER_WG)
+    k_gen_channel_cw(uint64_t seed, uint64_t first_cw, uint32_t n_cw, float mu, float sigma,
+                     uint16_t *__restrict__ out, const uint32_t *__restrict__ gen) {
+    __shared__ __attribute__((aligned(16))) uint16_t smem[WAVES_PER_WG][NV + 2 * KINFO / 32];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const uint32_t c = blockIdx.x * WAVES_PER_WG + wv;
+    if (c >= n_cw) return;
+    uint16_t *Ls = smem[wv];
+    wave_gen_channel<true>(Ls, reinterpret_cast<uint32_t *>(Ls + NV), lane, seed, first_cw + c, mu, sigma, gen);
+    wave_lds_fence();
+    uint4 *dst = reinterpret_cast<uint4 *>(out + (size_t)c * NV);
+#pragma unroll
+    for (int i = 0; i < NV / 8 / 64; ++i) dst[i * 64 + lane] = reinterpret_cast<const uint4 *>(Ls)[i * 64 + lane];
 }
 
 __global__ void __launch_bounds__(256)
@@ -372,6 +537,49 @@ __global__ void __launch_bounds__(256)
     uint16_t *L = out + (size_t)c * NV;
     *reinterpret_cast<uint2 *>(L + 4 * blk) = make_uint2(lo, hi);
     if (blk < (NV - NTX) / 4) *reinterpret_cast<uint2 *>(L + NTX + 4 * blk) = make_uint2(0u, 0u);
+}
+
+// Tail counts of the channel's normals: counts[k] += number of z with
+// |z| > thr[k] among the normals of codewords first_cw .. first_cw + n_cw - 1
+// (Philox blocks 0 .. n_tx/4 - 1 of each, exactly the z the channel of those
+// codewords uses).  Grid-stride over (codeword, block) pairs; per-thread
+// counts are summed over the wave with ballots and added once per wave.
+struct TailThresholds {
+    float thr[NOISE_TAIL_MAX];
+    int n;
+};
+
+__global__ void __launch_bounds__(256)
+    k_noise_tails(uint64_t seed, uint64_t first_cw, uint64_t n_cw, uint32_t blocks_per_cw, TailThresholds th,
+                  unsigned long long *counts) {
+    uint32_t local[NOISE_TAIL_MAX];
+#pragma unroll
+    for (int k = 0; k < NOISE_TAIL_MAX; ++k) local[k] = 0u;
+    const uint64_t total = n_cw * blocks_per_cw;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total; g += stride) {
+        const uint64_t w = first_cw + g / blocks_per_cw;
+        const uint32_t blk = (uint32_t)(g % blocks_per_cw);
+        const u32x4 x = philox4x32_10((uint32_t)w, (uint32_t)(w >> 32), blk, 0u, (uint32_t)seed,
+                                      (uint32_t)(seed >> 32));
+        float z[4];
+        box_muller(x.x, x.y, z[0], z[1]);
+        box_muller(x.z, x.w, z[2], z[3]);
+#pragma unroll
+        for (int k = 0; k < NOISE_TAIL_MAX; ++k)
+            if (k < th.n)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) local[k] += fabsf(z[j]) > th.thr[k] ? 1u : 0u;
+    }
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int k = 0; k < NOISE_TAIL_MAX; ++k) {
+        if (k >= th.n) break;
+        // wave sum of local[k] bit by bit (each lane's count < 2^24 here)
+        unsigned long long sum = 0;
+        for (int b = 0; b < 24; ++b) sum += (unsigned long long)__popcll(__ballot((local[k] >> b) & 1u)) << b;
+        if (lane == 0 && sum) atomicAdd(counts + k, sum);
+    }
 }
 
 __global__ void k_fill_u64(unsigned long long *p, size_t n, unsigned long long v) {
@@ -421,10 +629,41 @@ hipError_t t12_launch_decode(const uint16_t *llr_in, uint16_t *llr_out, uint32_t
 
 hipError_t t12_launch_mc(uint64_t seed, uint64_t first_cw, uint32_t n_cw, float mu, float sigma, int max_iter,
                          float beta, uint32_t *queue, unsigned long long *counters, unsigned long long *hist, int grid,
-                         const Capture &cap, hipStream_t st) {
+                         const Capture &cap, const uint32_t *cw_gen, hipStream_t st) {
     const float beta3 = beta < 1.0f ? beta : 1.0f;
-    hipLaunchKernelGGL(k_monte_carlo, dim3(grid), dim3(64 * WAVES_PER_WG), 0, st, seed, first_cw, n_cw, mu, sigma,
-                       max_iter, beta, beta3, queue, counters, hist, cap);
+    if (cw_gen)
+        hipLaunchKernelGGL(k_monte_carlo_cw, dim3(grid), dim3(64 * WAVES_PER_WG), 0, st, seed, first_cw, n_cw, mu,
+                           sigma, max_iter, beta, beta3, queue, counters, hist, cap, cw_gen);
+    else
+        hipLaunchKernelGGL(k_monte_carlo, dim3(grid), dim3(64 * WAVES_PER_WG), 0, st, seed, first_cw, n_cw, mu,
+                           sigma, max_iter, beta, beta3, queue, counters, hist, cap);
+    return hipGetLastError();
+}
+
+hipError_t t12_launch_gen_cw(uint64_t seed, uint64_t first_cw, uint32_t n_cw, float mu, float sigma, uint16_t *out,
+                             const uint32_t *cw_gen, hipStream_t st) {
+    if (!n_cw) return hipSuccess;
+    const unsigned grid = (n_cw + WAVES_PER_WG - 1) / WAVES_PER_WG;
+    hipLaunchKernelGGL(k_gen_channel_cw, dim3(grid), dim3(64 * WAVES_PER_WG), 0, st, seed, first_cw, n_cw, mu, sigma,
+                       out, cw_gen);
+    return hipGetLastError();
+}
+
+int t12_cw_gen_words() { return CW_GEN_WORDS; }
+
+hipError_t launch_noise_tails(uint64_t seed, uint64_t first_cw, uint64_t n_cw, int n_tx, const float *thr, int n_thr,
+                             unsigned long long *counts, int cus, hipStream_t st) {
+    if (n_thr < 1 || n_thr > NOISE_TAIL_MAX || n_tx < 4 || (n_tx & 3)) return hipErrorInvalidValue;
+    TailThresholds th{};
+    for (int k = 0; k < n_thr; ++k) th.thr[k] = thr[k];
+    th.n = n_thr;
+    const uint64_t total = n_cw * (uint64_t)(n_tx / 4);
+    // at most 2^19 Philox blocks (2^21 normals) per thread keeps every count < 2^24
+    uint64_t grid = (uint64_t)cus * 32;
+    while (grid * 256 * (1ull << 19) < total) grid *= 2;
+    if (!total) return hipSuccess;
+    hipLaunchKernelGGL(k_noise_tails, dim3((unsigned)grid), dim3(256), 0, st, seed, first_cw, n_cw,
+                       (uint32_t)(n_tx / 4), th, counts);
     return hipGetLastError();
 }
 

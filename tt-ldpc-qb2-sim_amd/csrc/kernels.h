@@ -27,12 +27,23 @@ hipError_t t12_occupancy(int *wg_per_cu);
 hipError_t t12_occupancy_decode(int *wg_per_cu);
 hipError_t t12_launch_decode(const uint16_t *llr_in, uint16_t *llr_out, uint32_t *iters, uint8_t *ok, uint32_t n_cw,
                              int max_iter, float beta, uint32_t *queue, int grid, hipStream_t st);
+// cw_gen: null for the all-zero codeword; else random-info-bit codewords with
+// the quasi-cyclic generator words of the 8 transmitted parity blocks in
+// device memory ([8][8][4] uint32, g_{r,i} of ldpc_get_generator for r < 8).
 hipError_t t12_launch_mc(uint64_t seed, uint64_t first_cw, uint32_t n_cw, float mu, float sigma, int max_iter,
                          float beta, uint32_t *queue, unsigned long long *counters, unsigned long long *hist, int grid,
-                         const Capture &cap, hipStream_t st);
+                         const Capture &cap, const uint32_t *cw_gen, hipStream_t st);
+hipError_t t12_launch_gen_cw(uint64_t seed, uint64_t first_cw, uint32_t n_cw, float mu, float sigma, uint16_t *out,
+                             const uint32_t *cw_gen, hipStream_t st);
+int t12_cw_gen_words();
 hipError_t t12_launch_gen(uint64_t seed, uint64_t first_cw, uint32_t n_cw, float mu, float sigma, uint16_t *out,
                           hipStream_t st);
 int t12_schedule(int32_t *out, int cap);
+// Tail counts of the channel normals (|z| > thr[k], k < n_thr <= NOISE_TAIL_MAX)
+// of codewords first_cw .. first_cw + n_cw - 1 with n_tx symbols each; adds to counts.
+constexpr int NOISE_TAIL_MAX = 16;
+hipError_t launch_noise_tails(uint64_t seed, uint64_t first_cw, uint64_t n_cw, int n_tx, const float *thr, int n_thr,
+                              unsigned long long *counts, int cus, hipStream_t st);
 
 // Profile R (decode_r.hip): list-scheduled rows, one codeword per wave, one
 // row per lane and pass.

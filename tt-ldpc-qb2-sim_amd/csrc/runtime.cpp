@@ -51,10 +51,15 @@ struct ldpc_ctx {
     float *d_alpha = nullptr, *d_beta = nullptr;
     int d_groups = 0, d_corr = 0;
     float d_rmax = 0.0f;
+    // random-info-bit codewords (ldpc_set_codeword_mode): generator words of
+    // the transmitted parity blocks on the device; null = the all-zero codeword
+    uint32_t *d_cw_gen = nullptr;
+    bool cw_random = false;
     // Profile R: level-ordered pass table on the device
     void *d_table = nullptr;
     ldpc::RLaunch rl{};
-    DevBuf in, out, iters, ok;
+    DevBuf in, out, iters, ok, tails;
+    int cus = 0;
     std::string err;
 };
 
@@ -434,6 +439,7 @@ int ldpc_create(const ldpc_code_desc *code, const ldpc_decoder_cfg *cfg, int dev
     int cus = 0;
     if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device)) != hipSuccess)
         return bail(e, "hipDeviceGetAttribute");
+    c->cus = cus;
 
     if (prof_d) {
         c->mv = ldpc::profile_d_kernel(k_id, rate_id);
@@ -499,9 +505,10 @@ void ldpc_destroy(ldpc_ctx *c) {
     // Teardown is best effort: errors here have no caller to report to.
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    for (DevBuf *b : {&c->in, &c->out, &c->iters, &c->ok})
+    for (DevBuf *b : {&c->in, &c->out, &c->iters, &c->ok, &c->tails})
         if (b->p) (void)hipFree(b->p);
     if (c->d_table) (void)hipFree(c->d_table);
+    if (c->d_cw_gen) (void)hipFree(c->d_cw_gen);
     if (c->d_alpha) (void)hipFree(c->d_alpha);
     if (c->d_cap_count) (void)hipFree(c->d_cap_count);
     if (c->d_cap_ids) (void)hipFree(c->d_cap_ids);
@@ -626,6 +633,54 @@ int ldpc_set_dmin_params(ldpc_ctx *c, const float *alpha, const float *beta, uin
     return LDPC_OK;
 }
 
+int ldpc_noise_tails(ldpc_ctx *c, uint64_t seed, uint64_t first_cw, uint64_t n_cw, const float *thresholds,
+                     uint32_t n_thr, uint64_t *counts) {
+    if (!c) return LDPC_ERR_ARG;
+    if (!thresholds || !counts || n_thr < 1 || n_thr > (uint32_t)ldpc::NOISE_TAIL_MAX)
+        return fail(c, LDPC_ERR_ARG, "thresholds: 1..16 values and a counts array are required");
+    for (uint32_t k = 0; k < n_thr; ++k)
+        if (!std::isfinite(thresholds[k])) return fail(c, LDPC_ERR_ARG, "thresholds must be finite");
+    if (n_cw > (1ull << 40)) return fail(c, LDPC_ERR_ARG, "n_cw must be <= 2^40");
+    int rc;
+    if ((rc = ensure(c, c->tails, ldpc::NOISE_TAIL_MAX * sizeof(unsigned long long)))) return rc;
+    HIP_TRY(c, hipSetDevice(c->device));
+    auto *d = static_cast<unsigned long long *>(c->tails.p);
+    HIP_TRY(c, ldpc::launch_fill_u64(d, n_thr, 0ull, c->stream));
+    HIP_TRY(c, ldpc::launch_noise_tails(seed, first_cw, n_cw, (int)c->info.n_tx, thresholds, (int)n_thr, d, c->cus,
+                                        c->stream));
+    HIP_TRY(c, hipMemcpyAsync(counts, d, n_thr * sizeof(uint64_t), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return LDPC_OK;
+}
+
+int ldpc_set_codeword_mode(ldpc_ctx *c, uint32_t mode) {
+    if (!c) return LDPC_ERR_ARG;
+    if (mode == LDPC_CODEWORD_ZERO) {
+        c->cw_random = false;
+        return LDPC_OK;
+    }
+    if (mode != LDPC_CODEWORD_RANDOM) return fail(c, LDPC_ERR_ARG, "unknown codeword mode");
+    if (c->cfg.profile != LDPC_PROFILE_T || c->mv || !is_r12_k1024(c->code))
+        return fail(c, LDPC_ERR_UNSUPPORTED,
+                    "random codewords are built into the Profile T kernel of AR4JA r1/2 K=1024 (variant 0)");
+    const uint32_t *w;
+    int z, nu;
+    std::string why;
+    const int rc = ldpc::qc_generator(0, 0, &w, &z, &nu, why);
+    if (rc) return fail(c, rc, why);
+    // transmitted parity blocks r = 0..7 of [12][nu = 8][z / 32 = 4]
+    const size_t nb = sizeof(uint32_t) * (size_t)ldpc::t12_cw_gen_words();
+    if (8 * nu * (z / 32) != ldpc::t12_cw_gen_words()) return fail(c, LDPC_ERR_HIP, "generator shape");
+    HIP_TRY(c, hipSetDevice(c->device));
+    if (!c->d_cw_gen) {
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        HIP_TRY(c, hipMalloc(&c->d_cw_gen, nb));
+        HIP_TRY(c, hipMemcpy(c->d_cw_gen, w, nb, hipMemcpyHostToDevice));
+    }
+    c->cw_random = true;
+    return LDPC_OK;
+}
+
 int ldpc_sync(ldpc_ctx *c) {
     if (!c) return LDPC_ERR_ARG;
     HIP_TRY(c, hipSetDevice(c->device));
@@ -692,7 +747,8 @@ int ldpc_run_point(ldpc_ctx *c, const ldpc_point_cfg *pt, ldpc_point_result *res
             HIP_TRY(c, c->mv->launch(p, c->grid_mc, c->stream));
         } else if (prof_t)
             HIP_TRY(c, ldpc::t12_launch_mc(pt->seed, next, (uint32_t)n, mu, sigma, (int)c->cfg.max_iter, c->cfg.beta,
-                                           c->d_queue, c->d_counters, c->d_hist, c->grid_mc, rl.cap, c->stream));
+                                           c->d_queue, c->d_counters, c->d_hist, c->grid_mc, rl.cap,
+                                           c->cw_random ? c->d_cw_gen : nullptr, c->stream));
         else
             HIP_TRY(c, ldpc::r_launch_mc(rl, pt->seed, next, (uint32_t)n, mu, sigma, c->d_counters, c->d_hist,
                                          c->stream));
@@ -786,6 +842,9 @@ int ldpc_gen_channel(ldpc_ctx *c, uint64_t seed, uint64_t first_cw, uint32_t n_c
     if (c->mv)  // same Philox / Box-Muller stream, size-generic generator, no clamp
         HIP_TRY(c, ldpc::r_launch_gen(seed, first_cw, n_cw, mu, sigma, 0.0f, (int)c->info.n_tx, (int)c->info.n,
                                       (uint16_t *)c->out.p, c->stream));
+    else if (c->cfg.profile == LDPC_PROFILE_T && c->cw_random)
+        HIP_TRY(c, ldpc::t12_launch_gen_cw(seed, first_cw, n_cw, mu, sigma, (uint16_t *)c->out.p, c->d_cw_gen,
+                                           c->stream));
     else if (c->cfg.profile == LDPC_PROFILE_T)
         HIP_TRY(c, ldpc::t12_launch_gen(seed, first_cw, n_cw, mu, sigma, (uint16_t *)c->out.p, c->stream));
     else

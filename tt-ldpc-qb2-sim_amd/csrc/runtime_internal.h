@@ -19,6 +19,9 @@ const MultiVariant *profile_d_kernel(int k_id, int rate_id);
 void fill_code_info(int k_id, int rate_id, ldpc_code_info &info);
 // BPSK LLR mean / deviation at ebn0_db for rate k_info / n_tx.
 void channel_params(const ldpc_code_info &info, double ebn0_db, float *mu, float *sigma);
+// Quasi-cyclic generator of an AR4JA code (encoder.cpp, cached): words =
+// [12 parity blocks][nu = K / z info blocks][z / 32] coefficient bits of g_{r,i}.
+int qc_generator(int k_id, int rate_id, const uint32_t **words, int *z, int *nu, std::string &why);
 // Message that ldpc_last_error(NULL) returns, for calls that take no context.
 void set_context_free_error(const std::string &msg);
 

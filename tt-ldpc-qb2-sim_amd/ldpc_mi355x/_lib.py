@@ -21,6 +21,8 @@ LDPC_PROFILE_T = 0
 LDPC_PROFILE_R = 1
 LDPC_PROFILE_R_NMS = 2
 LDPC_PROFILE_D = 3
+LDPC_CODEWORD_ZERO = 0
+LDPC_CODEWORD_RANDOM = 1
 LDPC_CODE_AR4JA = 1
 LDPC_CODE_MATRIX = 2
 
@@ -91,12 +93,17 @@ SIGNATURES = {
                                                 C.c_void_p]),
     "ldpc_decode_llrs_device_noalloc": (C.c_int, [C.POINTER(CodeDesc), C.POINTER(DecoderCfg), C.c_int, C.c_void_p,
                                                   C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "ldpc_get_generator": (C.c_int, [C.POINTER(CodeDesc), C.c_void_p, C.c_int32]),
+    "ldpc_encode": (C.c_int, [C.POINTER(CodeDesc), C.c_void_p, C.c_uint32, C.c_void_p]),
     "ldpc_gen_channel": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, C.c_uint32, C.c_double, C.c_void_p]),
     "ldpc_channel_params": (C.c_int, [C.c_void_p, C.c_double, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     "ldpc_set_capture": (C.c_int, [C.c_void_p, C.c_uint32]),
     "ldpc_get_captures": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.POINTER(C.c_uint64)]),
     "ldpc_set_dmin_params": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_int,
                                        C.c_float]),
+    "ldpc_set_codeword_mode": (C.c_int, [C.c_void_p, C.c_uint32]),
+    "ldpc_noise_tails": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, C.c_uint64, C.c_void_p, C.c_uint32,
+                                   C.c_void_p]),
     "ldpc_sync": (C.c_int, [C.c_void_p]),
     "ldpc_stream": (C.c_void_p, [C.c_void_p]),
 }

@@ -20,7 +20,7 @@ CSRC = os.path.join(ROOT, "csrc")
 LIB = os.path.join(PKG, "libldpc.so")
 ARCH = os.environ.get("LDPC_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["decode_t.hip", "decode_r.hip", "runtime.cpp", "batch.cpp", "noalloc.hip"]
+SOURCES = ["decode_t.hip", "decode_r.hip", "runtime.cpp", "batch.cpp", "noalloc.hip", "encoder.cpp"]
 # decode_multi.hip is compiled once per AR4JA code (K id, rate id)
 MULTI = "decode_multi.hip"
 MULTI_CODES = [(k, r) for k in range(3) for r in range(3)]

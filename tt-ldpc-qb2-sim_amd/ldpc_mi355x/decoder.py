@@ -15,6 +15,7 @@ from typing import Optional
 import numpy as np
 
 from . import _lib as L
+from . import codes
 
 RATE_FRACTIONS = {"1/2": (1, 2), "2/3": (2, 3), "4/5": (4, 5)}
 
@@ -80,6 +81,34 @@ def row_passes(rows, n, width=64):
     npass = lib.ldpc_row_passes(n, len(rows), rp.ctypes.data, col.ctypes.data, width, out.ctypes.data)
     if npass < 0:
         raise L.LdpcError(npass, "invalid matrix")
+    return out
+
+
+def generator(k_info: int, rate: str) -> np.ndarray:
+    """Quasi-cyclic generator of an AR4JA code as uint32 words
+    [12 parity blocks][k_info / z info blocks][z / 32] (ldpc_get_generator):
+    parity block r = sum_i g_{r,i} u_i mod x^z + 1."""
+    lib = L.load()
+    d = ar4ja_desc(k_info, rate)
+    n = lib.ldpc_get_generator(C.byref(d), None, 0)
+    if n < 0:
+        raise L.LdpcError(n, "no generator for this code")
+    out = np.zeros(n, dtype=np.uint32)
+    lib.ldpc_get_generator(C.byref(d), out.ctypes.data, n)
+    z = codes.ar4ja(k_info, rate).z
+    return out.reshape(12, k_info // z, z // 32)
+
+
+def encode(k_info: int, rate: str, info: np.ndarray) -> np.ndarray:
+    """Systematic codewords [n_cw, n] (bits as uint8) of info bits [n_cw, k_info] (ldpc_encode)."""
+    lib = L.load()
+    d = ar4ja_desc(k_info, rate)
+    info = np.ascontiguousarray(info, dtype=np.uint8).reshape(-1, k_info)
+    n = codes.ar4ja(k_info, rate).n
+    out = np.zeros((info.shape[0], n), dtype=np.uint8)
+    rc = lib.ldpc_encode(C.byref(d), info.ctypes.data, info.shape[0], out.ctypes.data)
+    if rc:
+        raise L.LdpcError(rc, "ldpc_encode failed (info bits must be 0 or 1)")
     return out
 
 
@@ -285,6 +314,20 @@ class Engine:
 
     def decode_llrs_device_async(self, d_llr: int, n_cw: int, d_out: int, d_iters: int, d_ok: int) -> None:
         self._check(self._lib.ldpc_decode_llrs_device_async(self._h, d_llr, n_cw, d_out, d_iters, d_ok))
+
+    def noise_tails(self, seed: int, first_cw: int, n_cw: int, thresholds) -> np.ndarray:
+        """Counts of channel normals with |z| > each threshold (ldpc_noise_tails)."""
+        thr = np.ascontiguousarray(thresholds, dtype=np.float32)
+        out = np.zeros(len(thr), dtype=np.uint64)
+        self._check(self._lib.ldpc_noise_tails(self._h, seed, first_cw, n_cw, thr.ctypes.data, len(thr),
+                                               out.ctypes.data))
+        return out
+
+    def set_codeword_mode(self, random_info: bool) -> None:
+        """All-zero codeword (the reference's only mode) or random info bits,
+        systematically encoded, for run_point and gen_channel."""
+        self._check(self._lib.ldpc_set_codeword_mode(
+            self._h, L.LDPC_CODEWORD_RANDOM if random_info else L.LDPC_CODEWORD_ZERO))
 
     def sync(self) -> None:
         self._check(self._lib.ldpc_sync(self._h))

@@ -74,6 +74,9 @@ def build_parser() -> argparse.ArgumentParser:
     p.add_argument("--seed", type=int, default=133742, help="Philox key of the channel stream (default: 133742)")
     p.add_argument("--beta", type=float, default=0.9375, help="Profile T damping (circulant engine)")
     p.add_argument("--device", type=int, default=0, help="HIP device index")
+    p.add_argument("--random_info", action="store_true",
+                   help="circulant engine: send random information bits, encoded, instead of the all-zero "
+                        "codeword, and count errors against the sent bits")
     return p
 
 
@@ -200,7 +203,12 @@ def make_engine(args, N, M):
     if args.engine == "circulant":
         if (N, M, args.punctured) != (2560, 1536, 512):
             raise SystemExit("--engine circulant decodes the AR4JA r1/2 K=1024 code (N=2560, M=1536, 512 punctured)")
-        return decoder.Engine(1024, "1/2", max_iter=args.max_iter, beta=args.beta, profile="T", device=args.device)
+        eng = decoder.Engine(1024, "1/2", max_iter=args.max_iter, beta=args.beta, profile="T", device=args.device)
+        if args.random_info:
+            eng.set_codeword_mode(True)
+        return eng
+    if args.random_info:
+        raise SystemExit("--random_info is built for the circulant engine (Profile T, AR4JA r1/2 K=1024)")
     with open(args.chinn) as f:
         first = next((ln for ln in f if ln.strip() and not ln.lstrip().startswith("#")), "")
     rows = codes.read_chinn_out(args.chinn)[2] if len(first.split()) == 3 else read_chinn_rows(args.chinn)
@@ -221,6 +229,8 @@ def read_chinn_rows(path):
 
 def main(argv=None):
     args = build_parser().parse_args(argv)
+    if args.random_info and args.binary:
+        raise SystemExit("--random_info runs in process (the engines' argv protocol has no such flag)")
     total_cores = 1 if args.single_core else CORES_MESH
     if args.max_blocks is not None:
         args.max_blocks_per_core = max(1, args.max_blocks // total_cores)
@@ -233,6 +243,8 @@ def main(argv=None):
     algo_desc = "Normalized Min-Sum (alpha=0.75)" if args.nms else "Approximate-Min* (Christopher Jones MILCOM 2003)"
     if args.engine == "circulant":
         algo_desc = f"Tree Approximate-Min* (golden circulant arithmetic, beta={args.beta})"
+        if args.random_info:
+            algo_desc += ", random info bits"
     print("==========================================================")
     print("         MI355X LDPC Monte Carlo Simulation               ")
     print("==========================================================")

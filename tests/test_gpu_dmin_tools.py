@@ -71,3 +71,12 @@ def test_eval_profile_d_and_forward(trained):
     rows = {ln["decoder"]: ln for ln in lines if "decoder" in ln}
     assert set(rows) == {"trained_forward_T7", "approx_min_star_20"}
     assert rows["trained_forward_T7"]["max_iter"] == 7 and rows["trained_forward_T7"]["blocks"] == 512
+
+
+def test_profile_tool_reports_hbm_rooflines():
+    _, lines = run([os.path.join(TOOLS, "profile_dmin.py"), "--T", "3", "--batch", "512", "--ckpt-every", "5",
+                    "--reps", "1"])
+    r = lines[-1]
+    assert r["layer_steps"] == 36 and r["batch"] == 512
+    for phase in ("forward", "forward_with_checkpoints", "backward"):
+        assert r[phase]["seconds"] > 0 and 0 < r[phase]["frac_hbm"] < 1.5

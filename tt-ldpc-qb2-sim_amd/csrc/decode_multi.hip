@@ -112,6 +112,10 @@ struct Cfg {
     //    6% faster at 3.0 dB.  (K = 1024 r4/5 in wave mode reaches 3 waves only
     //    with a forced 168-VGPR budget, which cost more than it gained at
     //    3.5 dB, so it keeps every message in VGPRs.)
+    // Waves per SIMD the compiler must fit.  Rate 4/5 in wave mode with one
+    // check per thread (K = 1024 r4/5) compiles to 171 VGPRs, 2 waves per
+    // SIMD; held to 3 waves it fits in 167 with no spill.
+    static constexpr int MIN_WAVES = (LDPC_RID == 2 && WAVE && C == 1) ? 3 : 1;
     static constexpr bool LDS_R12 = !WAVE && THREADS == 1024 && C == 2;
     static constexpr bool LDS_R45 = LDPC_RID == 2 && C == 1 && !WAVE && THREADS <= 256;
     static constexpr int deg18_before(int li) {
@@ -387,7 +391,8 @@ __device__ __forceinline__ void hist_flush(const uint32_t *hist_s, const MultiPa
 // Wave mode
 // ---------------------------------------------------------------------------
 template <class CF>
-__global__ void __launch_bounds__(CF::THREADS) k_multi_wave(MultiParams P) {
+__global__ void __launch_bounds__(CF::THREADS) __attribute__((amdgpu_waves_per_eu(CF::MIN_WAVES)))
+k_multi_wave(MultiParams P) {
     __shared__ __attribute__((aligned(16))) uint16_t smem[CF::CW_PER_WG * CF::N];
     __shared__ uint32_t Rl_s[CF::NRL > 0 ? CF::NRL * CF::THREADS : 1];
     __shared__ uint32_t hist_s[HIST_LDS];

@@ -315,12 +315,12 @@ def bench_scalar(args) -> None:
         "vs_baseline": None, "dtype": "bf16",
         "data": "synthetic: all-zero codeword, BPSK, AWGN from Philox4x32-10 + Box-Muller generated on device",
         "config": {"workload": "Profile R on matrices/AR4JA_r45_4c_128c_r12.chinn.out (rows in file order, run "
-                               "in 37 passes of <= 64), 512 punctured, Eb/N0 2.0 dB, 200 max iterations, 2^20 "
-                               "codewords per step",
+                               "two codewords per wave in 50 one-degree passes of <= 32 rows), 512 punctured, "
+                               "Eb/N0 2.0 dB, 200 max iterations, 2^20 codewords per step",
                    "max_iter": MAX_ITER, "codewords_per_step": CW_PER_STEP},
         "roofline": {"bound": "valu", "achieved": ops / per_launch_s / 1e12, "peak": PEAK_VALU_TOPS,
                      "unit": "Tlane-op/s", "frac": ops / per_launch_s / 1e12 / PEAK_VALU_TOPS, "traffic": None,
-                     "kernel": "k_monte_carlo_r", "kernel_ms_per_launch": per_launch_s * 1e3},
+                     "kernel": "k_r2", "kernel_ms_per_launch": per_launch_s * 1e3},
         "cpu_baseline": cpu,
         "fer": acc.fer, "fer_ci95": [fe_lo / acc.blocks, fe_hi / acc.blocks], "frame_errors": acc.frame_errors,
         "ber": acc.bit_errors / (acc.blocks * N_TX), "mean_iterations": acc.mean_iterations, "blocks": acc.blocks,

@@ -189,6 +189,16 @@ int ldpc_row_levels(uint32_t n, uint32_t m, const int32_t *row_ptr, const int32_
 int ldpc_row_passes(uint32_t n, uint32_t m, const int32_t *row_ptr, const int32_t *col, int32_t width,
                     int32_t *pass);
 
+/* Pass of every row in the two-codeword Profile R kernel's schedule (needs no
+ * GPU; no reference counterpart, like ldpc_row_passes).  Same readiness rule,
+ * passes of at most `width` rows (the kernel uses 32).  When every row has
+ * degree 3 or 6 each pass holds rows of one degree (kind 3 or 6); otherwise
+ * every pass has kind 7 (rows of any degree, masked datapath).  Writes m pass
+ * indices and, when kind is not NULL and kind_cap covers them, one kind per
+ * pass; returns the number of passes, or LDPC_ERR_ARG. */
+int ldpc_row_passes_by_degree(uint32_t n, uint32_t m, const int32_t *row_ptr, const int32_t *col, int32_t width,
+                              int32_t *pass, uint8_t *kind, int32_t kind_cap);
+
 /* Monte Carlo point: generate, decode and tally codewords first_cw.. until
  * max_blocks or target_fe is reached, in launches of chunk_cw codewords.
  * hist (may be NULL) has max_iter + 1 entries: hist[i] = codewords that

@@ -98,6 +98,69 @@ def test_monte_carlo_counters_exact(chinn, oracle, profile, ebn0, n, kw):
     np.testing.assert_array_equal(r.hist, o["hist"])
 
 
+def _random_rows(rng, n, degs):
+    return [sorted(rng.choice(n, size=int(d), replace=False).tolist()) for d in degs]
+
+
+@pytest.mark.parametrize("mix", ["exact36", "masked"])
+def test_two_codeword_kernel_on_random_matrices(oracle, mix):
+    """Random matrices through the two-codeword kernel: rows of degree 3 and 6
+    (one-degree passes) and of degree 1..6 (masked passes, partly filled).
+    Odd codeword counts leave a half without work at the end.  Decode and
+    Monte Carlo are bit-exact against the oracle, for Approx-Min* and NMS."""
+    from ldpc_mi355x import decoder
+    rng = np.random.default_rng(11 if mix == "exact36" else 12)
+    n, m = 700, 350
+    degs = rng.choice([3, 6], size=m) if mix == "exact36" else rng.integers(1, 7, size=m)
+    rows = _random_rows(rng, n, degs)
+    ps, kind = decoder.row_passes_by_degree(rows, n, 32)
+    assert ps.max() + 1 <= 64 and (set(kind.tolist()) <= {3, 6} if mix == "exact36" else set(kind.tolist()) == {7})
+    mu, sg = channel_params(1.5)
+    llr = oracle.gen_channel_clamped(n, n, 0x51, 0, 37, mu, sg)
+    for nms in (False, True):
+        _case(f"two-codeword kernel {mix} nms={nms} decode 37")
+        with decoder.Engine(matrix=(rows, n), n_punct=0, profile="R_NMS" if nms else "R", max_iter=25) as eng:
+            out, iters, ok = eng.decode_llrs(llr)
+            _case(f"two-codeword kernel {mix} nms={nms} run_point 1001")
+            r = eng.run_point(1.5, 1001, seed=9, first_cw=3, chunk_cw=1001)
+            emu, esg = eng.channel_params(1.5)
+        o_out, o_it, o_ok = oracle.decode_r(n, rows, llr, 25, nms)
+        np.testing.assert_array_equal(out, o_out)
+        np.testing.assert_array_equal(iters, o_it)
+        np.testing.assert_array_equal(ok, o_ok)
+        o = oracle.run_point_r(n, rows, n, 9, 3, 1001, emu, esg, np.float32(0.0), 25, nms, 0.0, 0.0)
+        assert (r.blocks, r.frame_errors, r.bit_errors, r.iter_sum) == \
+            (o["blocks"], o["frame_errors"], o["bit_errors"], o["iter_sum"])
+        np.testing.assert_array_equal(r.hist, o["hist"])
+
+
+def test_one_codeword_kernel_still_exact(chinn, oracle, monkeypatch):
+    """LDPC_R_ONE_CW=1 keeps the one-codeword kernel (messages in LDS); it stays
+    bit-exact on the shipped matrix, and a matrix whose dependency chain needs
+    more than 64 passes falls back to it by itself."""
+    from ldpc_mi355x import decoder
+    M, N, rows = chinn
+    mu, sg = channel_params(1.5)
+    llr = oracle.gen_channel_clamped(N, 2048, 0xABD, 0, 130, mu, sg)
+    monkeypatch.setenv("LDPC_R_ONE_CW", "1")
+    _case("one-codeword kernel, shipped matrix")
+    with _engine(chinn, profile="R", max_iter=40) as eng:
+        out, iters, ok = eng.decode_llrs(llr)
+    o_out, o_it, _ = oracle.decode_r(N, rows, llr, 40)
+    np.testing.assert_array_equal(out, o_out)
+    np.testing.assert_array_equal(iters, o_it)
+    monkeypatch.delenv("LDPC_R_ONE_CW")
+    chain = [[i, i + 1, (i + 7) % 101] for i in range(100)]
+    assert decoder.row_passes_by_degree(chain, 101, 32)[0].max() + 1 > 64
+    llr = oracle.gen_channel_clamped(101, 101, 3, 0, 20, 2.0, 1.5)
+    _case("long-chain matrix")
+    with decoder.Engine(matrix=(chain, 101), n_punct=0, profile="R", max_iter=12) as eng:
+        out, iters, ok = eng.decode_llrs(llr)
+    o_out, o_it, _ = oracle.decode_r(101, chain, llr, 12)
+    np.testing.assert_array_equal(out, o_out)
+    np.testing.assert_array_equal(iters, o_it)
+
+
 def test_profile_r_on_ar4ja_families(oracle):
     """Profile R on makeH-order AR4JA codes, including rate 2/3 (degree 10,
     the 12-wide row table)."""

@@ -134,6 +134,65 @@ def test_r_passes_on_random_matrices():
                 assert max(ps) + 1 == m
 
 
+def _check_passes(rows, ps, width, tag):
+    assert np.bincount(ps).max() <= width, tag
+    last = {}
+    for i, r in enumerate(rows):
+        for v in r:
+            assert last.get(v, -1) < ps[i], (tag, i)
+            last[v] = ps[i]
+
+
+def test_r2_passes_by_degree_on_the_shipped_matrix(chinn, oracle):
+    """The two-codeword kernel's schedule of the .chinn.out: passes of at most
+    32 rows, each pass of one degree, the minimum 32 degree-6 passes plus 18
+    degree-3 ones; sequentially equivalent (per-variable pass order follows
+    file order), and the oracle's decode in that pass order equals the
+    file-order decode bit for bit."""
+    from conftest import channel_params
+    from ldpc_mi355x import decoder
+    M, N, rows = chinn
+    ps, kind = decoder.row_passes_by_degree(rows, N, 32)
+    assert len(kind) == ps.max() + 1 == 50
+    assert (kind == 6).sum() == 32 and (kind == 3).sum() == 18
+    for i, r in enumerate(rows):
+        assert len(r) == kind[ps[i]]
+    _check_passes(rows, ps, 32, "chinn")
+    mu, sg = channel_params(1.2)
+    llr = oracle.gen_channel_clamped(N, 2048, 23, 0, 48, mu, sg)
+    order = sorted(range(M), key=lambda i: (ps[i], i))
+    for nms in (False, True):
+        seq = oracle.decode_r(N, rows, llr, 30, nms=nms)
+        par = oracle.decode_r(N, [rows[i] for i in order], llr, 30, nms=nms)
+        for a, b in zip(seq, par):
+            np.testing.assert_array_equal(a, b)
+
+
+def test_r2_passes_by_degree_on_random_matrices():
+    """Random matrices whose rows all have degree 3 or 6 get one-degree passes
+    (kinds 3 and 6); any other degree mix gets kind-7 passes equal to the plain
+    list schedule.  Every schedule keeps the per-variable file order."""
+    from ldpc_mi355x import decoder
+    rng = np.random.default_rng(321)
+    for trial in range(30):
+        n = int(rng.integers(16, 300))
+        m = int(rng.integers(1, 200))
+        exact = trial % 2 == 0
+        degs = rng.choice([3, 6], size=m) if exact else rng.integers(1, 7, size=m)
+        rows = [sorted(rng.choice(n, size=int(d), replace=False).tolist()) for d in degs]
+        for width in (5, 32):
+            ps, kind = decoder.row_passes_by_degree(rows, n, width)
+            assert len(kind) == ps.max() + 1
+            _check_passes(rows, ps, width, (trial, width))
+            if exact:
+                assert set(kind.tolist()) <= {3, 6}
+                for i, r in enumerate(rows):
+                    assert len(r) == kind[ps[i]], (trial, width, i)
+            elif not set(int(d) for d in degs) <= {3, 6}:
+                assert set(kind.tolist()) == {7}
+                assert np.array_equal(ps, decoder.row_passes(rows, n, width))
+
+
 def test_pass_order_decode_equals_sequential_on_random_matrices(oracle):
     """On random matrices and random bf16 LLRs, the oracle's Profile R decode
     with the rows in the kernel's pass order (file order inside a pass)

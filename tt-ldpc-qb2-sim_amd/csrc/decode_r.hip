@@ -4,10 +4,7 @@
 // Bit-exact restatement of engines/scalar_5w/ldpc_decoder_core.h:
 //   q = L - R (fp32, unrounded)                                   (:422-436)
 //   sign parity and min / argmin, first index wins ties          (:424-440)
-//   recursive Approx-Min* with fp32 constants                    (:176-201)
-//     (the degree-3 variant :155-173 is the same function: for finite
-//      floats a > b <=> a - b > 0, so "(m > c) ? m - c : 0" and
-//      "max(m - c, 0)" agree)
+//   recursive Approx-Min* with fp32 constants (profile_r.h)      (:176-201)
 //   or normalized min-sum, alpha = 0.75 (USE_NORMALIZED_MIN_SUM) (:320-411)
 //   optional clamps |R| <= r_max, |L'| <= l_post_max             (:490-506)
 //   L' = bf16(q + r), R' = bf16(r)                                (:507-508)
@@ -30,6 +27,7 @@
 
 #include "device_math.h"
 #include "kernels.h"
+#include "profile_r.h"
 
 #pragma clang fp contract(off)
 
@@ -38,28 +36,10 @@ namespace {
 
 constexpr int WAVES = 2;  // 2 x 20 KiB of LDS per workgroup for r1/2 K=1024
 
-__device__ __forceinline__ float bf(uint16_t h) { return u2f((uint32_t)h << 16); }
-__device__ __forceinline__ float absbits(float x) { return u2f(f2u(x) & 0x7FFFFFFFu); }
+using pr::absbits;
+using pr::fold_r;
+using pr::RCfg;
 __device__ __forceinline__ void wave_fence() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront"); }
-
-// Correction of the recursive min* (ldpc_decoder_core.h:162):
-// 0 for diff >= 2; 0.69315 - 0.40 diff below 0.5; 0.49315 - 0.328 (diff - 0.5)
-__device__ __forceinline__ float corr_r(float diff) {
-    const float l1 = __fsub_rn(0.69315f, __fmul_rn(0.40f, diff));
-    const float l2 = __fsub_rn(0.49315f, __fmul_rn(0.328f, __fsub_rn(diff, 0.5f)));
-    const float c = diff < 0.5f ? l1 : l2;
-    return diff >= 2.0f ? 0.0f : c;
-}
-
-// One fold step: max(min(m, a) - g(|m - a|), 0).  m, a >= +0 and the
-// difference is never -0, so fmaxf agrees with the reference's select.
-// For non-negative finite floats the bit patterns order like the values, so
-// min(m, a) is an unsigned integer min (no canonicalisation of the inputs).
-__device__ __forceinline__ float fold_r(float m, float a) {
-    const float diff = fabsf(__fsub_rn(m, a));
-    const float cur = u2f(min(f2u(m), f2u(a)));
-    return fmaxf(__fsub_rn(cur, corr_r(diff)), 0.0f);
-}
 
 // One lane's entry of the pass table as loaded: word 0 = degree |
 // edge_start << 4, then the variable indices as 16-bit pairs (4 words per
@@ -89,10 +69,6 @@ __device__ __forceinline__ int var_of(const RowRaw<MAXD> &r, int d) {
     const uint32_t word = (k % 4) == 0 ? w.x : (k % 4) == 1 ? w.y : (k % 4) == 2 ? w.z : w.w;
     return (int)((d & 1) ? (word >> 16) : (word & 0xFFFFu));
 }
-
-struct RCfg {
-    float r_max, l_post_max;
-};
 
 // One row on one lane.  Straight-line code: all 2 * MAXD gathers first,
 // then the arithmetic with slots d >= deg masked by selects, then the
@@ -439,7 +415,10 @@ int r_waves_per_wg() { return WAVES; }
 
 size_t r_lds_bytes(int n, int n_edges) { return (size_t)WAVES * 2 * (((n + n_edges) + 7) & ~7); }
 
+size_t r_launch_lds(const RLaunch &L) { return L.r2_mode >= 0 ? r2_lds_bytes(L) : r_lds_bytes(L.n, L.n_edges); }
+
 hipError_t r_occupancy(const RLaunch &L, size_t lds, int *wg_per_cu) {
+    if (L.r2_mode >= 0) return r2_occupancy(L, wg_per_cu);
 #define OCC(M, N, C) occ_t<M, N, C>(lds, wg_per_cu)
     R_DISPATCH(L, OCC);
 #undef OCC
@@ -447,6 +426,7 @@ hipError_t r_occupancy(const RLaunch &L, size_t lds, int *wg_per_cu) {
 
 hipError_t r_launch_decode(const RLaunch &L, const uint16_t *llr_in, uint16_t *llr_out, uint32_t *iters, uint8_t *ok,
                            uint32_t n_cw, hipStream_t st) {
+    if (L.r2_mode >= 0) return r2_launch_decode(L, llr_in, llr_out, iters, ok, n_cw, st);
 #define DEC(M, N, C) launch_decode_t<M, N, C>(L, llr_in, llr_out, iters, ok, n_cw, st)
     R_DISPATCH(L, DEC);
 #undef DEC
@@ -454,6 +434,7 @@ hipError_t r_launch_decode(const RLaunch &L, const uint16_t *llr_in, uint16_t *l
 
 hipError_t r_launch_mc(const RLaunch &L, uint64_t seed, uint64_t first_cw, uint32_t n_cw, float mu, float sigma,
                        unsigned long long *counters, unsigned long long *hist, hipStream_t st) {
+    if (L.r2_mode >= 0) return r2_launch_mc(L, seed, first_cw, n_cw, mu, sigma, counters, hist, st);
 #define MC(M, N, C) launch_mc_t<M, N, C>(L, seed, first_cw, n_cw, mu, sigma, counters, hist, st)
     R_DISPATCH(L, MC);
 #undef MC

@@ -45,9 +45,21 @@ constexpr int NOISE_TAIL_MAX = 16;
 hipError_t launch_noise_tails(uint64_t seed, uint64_t first_cw, uint64_t n_cw, int n_tx, const float *thr, int n_thr,
                               unsigned long long *counts, int cus, hipStream_t st);
 
-// Profile R (decode_r.hip): list-scheduled rows, one codeword per wave, one
-// row per lane and pass.
+// Profile R: list-scheduled rows in passes.  Two kernels:
+//  * decode_r.hip: one codeword per wave, one row per lane and pass of up to
+//    R_PASS_ROWS rows, messages in LDS (any degree <= 12);
+//  * decode_r2.hip: two codewords per wave (one per 32-lane half), passes of
+//    up to R2_PASS_ROWS rows, messages compressed in VGPRs: degree <= 6 and at
+//    most R2_MAX_PASSES passes.  r2_mode >= 0 selects it; the r_launch_*
+//    entry points dispatch.
 constexpr int R_PASS_ROWS = 64;
+constexpr int R2_PASS_ROWS = 32;
+constexpr int R2_MAX_PASSES = 64;
+// pass kinds of the two-codeword kernel
+constexpr uint32_t R2_PK_NONE = 0;    // past the last pass
+constexpr uint32_t R2_PK_EXACT3 = 3;  // every row of the pass has degree 3
+constexpr uint32_t R2_PK_EXACT6 = 6;  // every row of the pass has degree 6
+constexpr uint32_t R2_PK_MASKED = 7;  // rows of degree 1..6, masked datapath
 struct RLaunch {
     const void *table;  // n_pass * R_PASS_ROWS entries of 16 B (maxd <= 6) or 32 B (maxd <= 12)
     int n_pass, n, n_tx, max_iter;
@@ -58,9 +70,17 @@ struct RLaunch {
     int grid;
     uint32_t *queue;
     Capture cap;
+    // two-codeword kernel
+    int r2_mode = -1;                  // -1: not used; 0: passes of kind EXACT3 / EXACT6; 1: MASKED passes
+    const void *table2 = nullptr;      // (4 n_groups + 1) * 64 entries of 16 B
+    const uint32_t *kinds2 = nullptr;  // n_groups words, byte j of word g = kind of pass 4 g + j
+    int n_groups = 0;                  // ceil(passes / 4)
+    int hs = 0;                        // bytes of LDS per half: the LLRs, a zero slot, padding
 };
 int r_waves_per_wg();
 size_t r_lds_bytes(int n, int n_edges);
+// LDS per workgroup of the kernel L selects
+size_t r_launch_lds(const RLaunch &L);
 hipError_t r_occupancy(const RLaunch &L, size_t lds, int *wg_per_cu);
 hipError_t r_launch_decode(const RLaunch &L, const uint16_t *llr_in, uint16_t *llr_out, uint32_t *iters, uint8_t *ok,
                            uint32_t n_cw, hipStream_t st);
@@ -68,6 +88,12 @@ hipError_t r_launch_gen(uint64_t seed, uint64_t first_cw, uint32_t n_cw, float m
                         int n, uint16_t *out, hipStream_t st);
 hipError_t r_launch_mc(const RLaunch &L, uint64_t seed, uint64_t first_cw, uint32_t n_cw, float mu, float sigma,
                        unsigned long long *counters, unsigned long long *hist, hipStream_t st);
+size_t r2_lds_bytes(const RLaunch &L);
+hipError_t r2_occupancy(const RLaunch &L, int *wg_per_cu);
+hipError_t r2_launch_decode(const RLaunch &L, const uint16_t *llr_in, uint16_t *llr_out, uint32_t *iters, uint8_t *ok,
+                            uint32_t n_cw, hipStream_t st);
+hipError_t r2_launch_mc(const RLaunch &L, uint64_t seed, uint64_t first_cw, uint32_t n_cw, float mu, float sigma,
+                        unsigned long long *counters, unsigned long long *hist, hipStream_t st);
 
 // Profile T for the nine AR4JA codes (decode_multi.hip, one object per code).
 // Per-launch values a captured graph changes between replays, read by the

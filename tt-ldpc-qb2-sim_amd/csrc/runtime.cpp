@@ -8,6 +8,7 @@
 #include <queue>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -55,8 +56,10 @@ struct ldpc_ctx {
     // the transmitted parity blocks on the device; null = the all-zero codeword
     uint32_t *d_cw_gen = nullptr;
     bool cw_random = false;
-    // Profile R: level-ordered pass table on the device
+    // Profile R: pass table on the device (and the pass kinds of the
+    // two-codeword kernel)
     void *d_table = nullptr;
+    uint32_t *d_kinds = nullptr;
     ldpc::RLaunch rl{};
     DevBuf in, out, iters, ok, tails;
     int cus = 0;
@@ -179,6 +182,114 @@ int schedule_passes(uint32_t n, uint32_t m, const int32_t *rp, const int32_t *co
     return n_pass;
 }
 
+// Rows in degree-homogeneous passes of at most `width`, for the two-codeword
+// Profile R kernel.  Same readiness rule as schedule_passes.  When every row
+// has degree 3 or 6, each pass takes ready rows of one degree only (kind
+// R2_PK_EXACT3 / R2_PK_EXACT6), and a degree-3 pass costs about 0.42 of a
+// degree-6 one (its datapath's instruction count).  The degree of each pass
+// is chosen by rollout: for each degree with ready rows, the rest of the
+// schedule is completed greedily (each pass takes the degree of the ready
+// row with the longest chain after it) and the cheaper total wins; for the
+// r1/2 K=1024 matrix with passes of 32 this gives 32 degree-6 passes (the
+// minimum) and 18 degree-3 ones.  Other matrices get plain list-scheduled
+// passes of kind R2_PK_MASKED.  Writes the pass of every row and the kind of
+// every pass; returns the number of passes (-1 on a scheduling failure).
+int schedule_passes_by_degree(uint32_t n, uint32_t m, const int32_t *rp, const int32_t *col, int width,
+                              int32_t *pass, std::vector<uint8_t> &kind) {
+    bool exact = true;
+    for (uint32_t i = 0; i < m; ++i) {
+        const int d = rp[i + 1] - rp[i];
+        exact = exact && (d == 3 || d == 6);
+    }
+    if (!exact) {
+        const int np = schedule_passes(n, m, rp, col, width, pass);
+        kind.assign(np > 0 ? np : 0, (uint8_t)ldpc::R2_PK_MASKED);
+        return np;
+    }
+    std::vector<int32_t> last(n, -1);
+    std::vector<std::vector<uint32_t>> succ(m);
+    std::vector<int32_t> indeg0(m, 0);
+    for (uint32_t i = 0; i < m; ++i)
+        for (int32_t e = rp[i]; e < rp[i + 1]; ++e) {
+            const int32_t p = last[col[e]];
+            if (p >= 0 && p != (int32_t)i) {
+                succ[p].push_back(i);
+                ++indeg0[i];
+            }
+            last[col[e]] = (int32_t)i;
+        }
+    std::vector<int32_t> height(m);
+    for (uint32_t k = m; k-- > 0;) {
+        int32_t x = 0;
+        for (uint32_t s : succ[k]) x = std::max(x, height[s]);
+        height[k] = x + 1;
+    }
+    auto later = [&](uint32_t a, uint32_t b) { return height[a] != height[b] ? height[a] < height[b] : a > b; };
+    using Heap = std::priority_queue<uint32_t, std::vector<uint32_t>, decltype(later)>;
+    auto cls_of = [&](uint32_t r) { return rp[r + 1] - rp[r] == 3 ? 0 : 1; };
+    const double cost[2] = {0.42, 1.0};
+    struct State {
+        std::vector<int32_t> indeg;
+        std::vector<Heap> ready;
+        uint32_t left;
+    };
+    State st{indeg0, {Heap(later), Heap(later)}, m};
+    for (uint32_t i = 0; i < m; ++i)
+        if (!indeg0[i]) st.ready[cls_of(i)].push(i);
+    // one pass of class c from state s; rows go to `out` when given
+    auto step = [&](State &s, int c, std::vector<uint32_t> *out) {
+        std::vector<uint32_t> cur;
+        while (!s.ready[c].empty() && (int)cur.size() < width) {
+            cur.push_back(s.ready[c].top());
+            s.ready[c].pop();
+        }
+        for (uint32_t r : cur)
+            for (uint32_t x : succ[r])
+                if (--s.indeg[x] == 0) s.ready[cls_of(x)].push(x);
+        s.left -= (uint32_t)cur.size();
+        if (out) *out = cur;
+    };
+    // class of the ready row with the longest chain after it (-1: none ready)
+    auto top_cls = [&](const State &s) {
+        if (s.ready[0].empty()) return s.ready[1].empty() ? -1 : 1;
+        if (s.ready[1].empty()) return 0;
+        return later(s.ready[0].top(), s.ready[1].top()) ? 1 : 0;
+    };
+    auto greedy_cost = [&](State s) {
+        double c = 0.0;
+        for (int k; s.left && (k = top_cls(s)) >= 0;) {
+            step(s, k, nullptr);
+            c += cost[k];
+        }
+        return s.left ? 1e30 : c;
+    };
+    // the rollout costs O(passes^2 * m); larger matrices (which the
+    // two-codeword kernel cannot hold anyway) take the greedy choice
+    const bool rollout = m <= (uint32_t)(ldpc::R2_MAX_PASSES * width);
+    kind.clear();
+    std::vector<uint32_t> rows;
+    while (st.left) {
+        int best = top_cls(st);
+        if (best < 0) return -1;
+        if (rollout && !st.ready[0].empty() && !st.ready[1].empty()) {
+            double bc = 1e31;
+            for (int k : {best, 1 - best}) {
+                State t = st;
+                step(t, k, nullptr);
+                const double c = cost[k] + greedy_cost(t);
+                if (c < bc) {
+                    bc = c;
+                    best = k;
+                }
+            }
+        }
+        step(st, best, &rows);
+        for (uint32_t r : rows) pass[r] = (int32_t)kind.size();
+        kind.push_back(best ? (uint8_t)ldpc::R2_PK_EXACT6 : (uint8_t)ldpc::R2_PK_EXACT3);
+    }
+    return (int)kind.size();
+}
+
 int validate_rows(uint32_t n, uint32_t m, const int32_t *rp, const int32_t *col, std::string &why) {
     if (!rp || !col || n == 0 || m == 0) {
         why = "matrix: empty or null arrays";
@@ -222,51 +333,107 @@ int setup_profile_r(ldpc_ctx *c, const std::vector<int32_t> &rp, const std::vect
     const uint32_t n_edges = (uint32_t)rp[m];
     int maxd = 0;
     for (uint32_t i = 0; i < m; ++i) maxd = std::max(maxd, rp[i + 1] - rp[i]);
-    const int kmaxd = maxd <= 6 ? 6 : 12;  // the kernel's slot count
-    // Message layout in LDS (private to the kernel, zeroed on entry): rows in
-    // file order, except that the last row of the largest degree is moved to
-    // the end, plus kmaxd - maxd spare slots.  Then es + kmaxd - 1 is a valid
-    // slot for every row, and the kernel reads all kmaxd slots unmasked.
-    const uint32_t r_slots = n_edges + (uint32_t)(kmaxd - maxd);
-    if (n_edges >= (1u << 28) || ldpc::r_lds_bytes((int)n, (int)r_slots) > 65536)
-        return fail(nullptr, LDPC_ERR_UNSUPPORTED,
-                    "Profile R keeps a codeword's LLRs and messages in LDS: needs n + edges <= 16384");
-    std::vector<uint32_t> r_off(m);
-    {
-        uint32_t last = 0;
-        for (uint32_t i = 0; i < m; ++i)
-            if (rp[i + 1] - rp[i] == maxd) last = i;
-        uint32_t off = 0;
-        for (uint32_t i = 0; i < m; ++i)
-            if (i != last) {
-                r_off[i] = off;
-                off += (uint32_t)(rp[i + 1] - rp[i]);
-            }
-        r_off[last] = off;
-    }
-    std::vector<int32_t> level(m), pass(m);
+    std::vector<int32_t> level(m);
     const int n_levels = levelize(n, m, rp.data(), col.data(), level.data());
-    const int n_pass = schedule_passes(n, m, rp.data(), col.data(), ldpc::R_PASS_ROWS, pass.data());
-    if (n_pass <= 0) return fail(nullptr, LDPC_ERR_ARG, "Profile R: rows could not be scheduled");
-    const int words = kmaxd == 6 ? 4 : 8;  // uint32 per row entry
-    // entry (p, k): the k-th row of pass p (file order inside a pass); unused
-    // entries stay zero (degree 0)
-    std::vector<uint32_t> table((size_t)n_pass * ldpc::R_PASS_ROWS * words, 0u);
-    std::vector<int> fill(n_pass, 0);
-    for (uint32_t r = 0; r < m; ++r) {
-        uint32_t *e = &table[((size_t)pass[r] * ldpc::R_PASS_ROWS + fill[pass[r]]++) * words];
-        const uint32_t deg = (uint32_t)(rp[r + 1] - rp[r]);
-        e[0] = deg | (r_off[r] << 4);
-        // variables 0..5 in words 1..3, 6..11 in words 4..6
-        for (uint32_t d = 0; d < deg; ++d) e[1 + d / 2] |= ((uint32_t)col[rp[r] + d] & 0xFFFFu) << (16 * (d & 1));
+    // The two-codeword kernel (decode_r2.hip) where the code fits it: degree
+    // <= 6, at most R2_MAX_PASSES passes of 32 rows, both codewords' LLRs in
+    // 64 KiB of LDS.  LDPC_R_ONE_CW=1 in the environment keeps the
+    // one-codeword kernel (for comparisons).
+    const char *one_cw = std::getenv("LDPC_R_ONE_CW");
+    const uint32_t hs = (2 * (n + 1) + 15) & ~15u;  // bytes of LDS per half: LLRs, zero slot at n
+    if (maxd <= 6 && 2 * hs <= 65536 && !(one_cw && one_cw[0] == '1')) {
+        std::vector<int32_t> pass(m);
+        std::vector<uint8_t> kind;
+        const int np2 = schedule_passes_by_degree(n, m, rp.data(), col.data(), ldpc::R2_PASS_ROWS, pass.data(), kind);
+        if (np2 > 0 && np2 <= ldpc::R2_MAX_PASSES) {
+            const int n_groups = (np2 + 3) / 4;
+            const size_t n_entries = (size_t)(4 * n_groups + 1) * 64;
+            // entry (p, lane): lane k < 32 runs the k-th row of pass p on the
+            // first codeword, lane k + 32 on the second (offsets + hs).  Word 0:
+            // degree (0 on idle lanes); words 1..3: LDS byte offsets of the
+            // row's variables, 16 bits each, the half's zero slot past the degree
+            std::vector<uint32_t> table(n_entries * 4, 0u);
+            const uint32_t zero = 2 * n;
+            for (size_t e = 0; e < n_entries; ++e) {
+                const uint32_t z = zero + (e % 64 >= 32 ? hs : 0);
+                for (int w = 1; w < 4; ++w) table[e * 4 + w] = z | (z << 16);
+            }
+            std::vector<int> fill(np2, 0);
+            for (uint32_t r = 0; r < m; ++r) {
+                const int k = fill[pass[r]]++;
+                const uint32_t deg = (uint32_t)(rp[r + 1] - rp[r]);
+                for (int h = 0; h < 2; ++h) {
+                    uint32_t *e = &table[((size_t)pass[r] * 64 + k + 32 * h) * 4];
+                    e[0] = deg;
+                    for (uint32_t d = 0; d < deg; ++d) {
+                        const uint32_t o = 2 * (uint32_t)col[rp[r] + d] + (h ? hs : 0);
+                        uint32_t &wd = e[1 + d / 2];
+                        wd = (d & 1) ? ((wd & 0xFFFFu) | (o << 16)) : ((wd & 0xFFFF0000u) | o);
+                    }
+                }
+            }
+            std::vector<uint32_t> kinds(n_groups, 0u);
+            for (int p = 0; p < np2; ++p) kinds[p / 4] |= (uint32_t)kind[p] << (8 * (p % 4));
+            HIP_TRY(c, hipMalloc(&c->d_table, table.size() * sizeof(uint32_t)));
+            HIP_TRY(c, hipMemcpy(c->d_table, table.data(), table.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+            HIP_TRY(c, hipMalloc(&c->d_kinds, kinds.size() * sizeof(uint32_t)));
+            HIP_TRY(c, hipMemcpy(c->d_kinds, kinds.data(), kinds.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+            c->rl.r2_mode = kind[0] == ldpc::R2_PK_MASKED ? 1 : 0;
+            c->rl.table2 = c->d_table;
+            c->rl.kinds2 = c->d_kinds;
+            c->rl.n_groups = n_groups;
+            c->rl.hs = (int)hs;
+            c->rl.n_pass = np2;
+            c->rl.n_edges = (int)n_edges;
+        }
     }
-    HIP_TRY(c, hipMalloc(&c->d_table, table.size() * sizeof(uint32_t)));
-    HIP_TRY(c, hipMemcpy(c->d_table, table.data(), table.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
-    c->rl.table = c->d_table;
-    c->rl.n_pass = n_pass;
+    if (c->rl.r2_mode < 0) {
+        const int kmaxd = maxd <= 6 ? 6 : 12;  // the kernel's slot count
+        // Message layout in LDS (private to the kernel, zeroed on entry): rows in
+        // file order, except that the last row of the largest degree is moved to
+        // the end, plus kmaxd - maxd spare slots.  Then es + kmaxd - 1 is a valid
+        // slot for every row, and the kernel reads all kmaxd slots unmasked.
+        const uint32_t r_slots = n_edges + (uint32_t)(kmaxd - maxd);
+        if (n_edges >= (1u << 28) || ldpc::r_lds_bytes((int)n, (int)r_slots) > 65536)
+            return fail(nullptr, LDPC_ERR_UNSUPPORTED,
+                        "Profile R keeps a codeword's LLRs and messages in LDS: needs n + edges <= 16384");
+        std::vector<uint32_t> r_off(m);
+        {
+            uint32_t last = 0;
+            for (uint32_t i = 0; i < m; ++i)
+                if (rp[i + 1] - rp[i] == maxd) last = i;
+            uint32_t off = 0;
+            for (uint32_t i = 0; i < m; ++i)
+                if (i != last) {
+                    r_off[i] = off;
+                    off += (uint32_t)(rp[i + 1] - rp[i]);
+                }
+            r_off[last] = off;
+        }
+        std::vector<int32_t> pass(m);
+        const int n_pass = schedule_passes(n, m, rp.data(), col.data(), ldpc::R_PASS_ROWS, pass.data());
+        if (n_pass <= 0) return fail(nullptr, LDPC_ERR_ARG, "Profile R: rows could not be scheduled");
+        const int words = kmaxd == 6 ? 4 : 8;  // uint32 per row entry
+        // entry (p, k): the k-th row of pass p (file order inside a pass); unused
+        // entries stay zero (degree 0)
+        std::vector<uint32_t> table((size_t)n_pass * ldpc::R_PASS_ROWS * words, 0u);
+        std::vector<int> fill(n_pass, 0);
+        for (uint32_t r = 0; r < m; ++r) {
+            uint32_t *e = &table[((size_t)pass[r] * ldpc::R_PASS_ROWS + fill[pass[r]]++) * words];
+            const uint32_t deg = (uint32_t)(rp[r + 1] - rp[r]);
+            e[0] = deg | (r_off[r] << 4);
+            // variables 0..5 in words 1..3, 6..11 in words 4..6
+            for (uint32_t d = 0; d < deg; ++d)
+                e[1 + d / 2] |= ((uint32_t)col[rp[r] + d] & 0xFFFFu) << (16 * (d & 1));
+        }
+        HIP_TRY(c, hipMalloc(&c->d_table, table.size() * sizeof(uint32_t)));
+        HIP_TRY(c, hipMemcpy(c->d_table, table.data(), table.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+        c->rl.table = c->d_table;
+        c->rl.n_pass = n_pass;
+        c->rl.n_edges = (int)r_slots;
+    }
     c->rl.n = (int)n;
     c->rl.n_tx = (int)(n - np);
-    c->rl.n_edges = (int)r_slots;
     c->rl.max_iter = (int)c->cfg.max_iter;
     c->rl.r_max = c->cfg.r_max;
     c->rl.l_post_max = c->cfg.l_post_max;
@@ -374,6 +541,20 @@ int ldpc_row_passes(uint32_t n, uint32_t m, const int32_t *row_ptr, const int32_
     std::string why;
     if (!pass || width < 1 || validate_rows(n, m, row_ptr, col, why)) return LDPC_ERR_ARG;
     return schedule_passes(n, m, row_ptr, col, width, pass);
+}
+
+int ldpc_row_passes_by_degree(uint32_t n, uint32_t m, const int32_t *row_ptr, const int32_t *col, int32_t width,
+                              int32_t *pass, uint8_t *kind, int32_t kind_cap) {
+    std::string why;
+    if (!pass || width < 1 || validate_rows(n, m, row_ptr, col, why)) return LDPC_ERR_ARG;
+    std::vector<uint8_t> k;
+    const int np = schedule_passes_by_degree(n, m, row_ptr, col, width, pass, k);
+    if (np < 0) return LDPC_ERR_ARG;
+    if (kind) {
+        if (kind_cap < np) return LDPC_ERR_ARG;
+        std::copy(k.begin(), k.end(), kind);
+    }
+    return np;
 }
 
 int ldpc_create(const ldpc_code_desc *code, const ldpc_decoder_cfg *cfg, int device, ldpc_ctx **out) {
@@ -489,7 +670,7 @@ int ldpc_create(const ldpc_code_desc *code, const ldpc_decoder_cfg *cfg, int dev
             return rc;
         }
         int occ = 0;
-        if ((e = ldpc::r_occupancy(c->rl, ldpc::r_lds_bytes(c->rl.n, c->rl.n_edges), &occ)) !=
+        if ((e = ldpc::r_occupancy(c->rl, ldpc::r_launch_lds(c->rl), &occ)) !=
             hipSuccess)
             return bail(e, "occupancy");
         c->grid_mc = c->grid_dec = cus * (occ > 0 ? occ : 1);
@@ -508,6 +689,7 @@ void ldpc_destroy(ldpc_ctx *c) {
     for (DevBuf *b : {&c->in, &c->out, &c->iters, &c->ok, &c->tails})
         if (b->p) (void)hipFree(b->p);
     if (c->d_table) (void)hipFree(c->d_table);
+    if (c->d_kinds) (void)hipFree(c->d_kinds);
     if (c->d_cw_gen) (void)hipFree(c->d_cw_gen);
     if (c->d_alpha) (void)hipFree(c->d_alpha);
     if (c->d_cap_count) (void)hipFree(c->d_cap_count);

@@ -87,6 +87,8 @@ SIGNATURES = {
     "ldpc_batch_destroy": (None, [C.c_void_p]),
     "ldpc_row_levels": (C.c_int, [C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]),
     "ldpc_row_passes": (C.c_int, [C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]),
+    "ldpc_row_passes_by_degree": (C.c_int, [C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p,
+                                            C.c_void_p, C.c_int32]),
     "ldpc_run_point": (C.c_int, [C.c_void_p, C.POINTER(PointCfg), C.POINTER(PointResult), C.c_void_p]),
     "ldpc_decode_llrs": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]),
     "ldpc_decode_llrs_device_async": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p,

@@ -20,13 +20,13 @@ CSRC = os.path.join(ROOT, "csrc")
 LIB = os.path.join(PKG, "libldpc.so")
 ARCH = os.environ.get("LDPC_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["decode_t.hip", "decode_r.hip", "runtime.cpp", "batch.cpp", "noalloc.hip", "encoder.cpp"]
+SOURCES = ["decode_t.hip", "decode_r.hip", "decode_r2.hip", "runtime.cpp", "batch.cpp", "noalloc.hip", "encoder.cpp"]
 # decode_multi.hip is compiled once per AR4JA code (K id, rate id)
 MULTI = "decode_multi.hip"
 MULTI_CODES = [(k, r) for k in range(3) for r in range(3)]
 # command-line shims speaking the reference engines' CLI, linked to libldpc
 CLIS = {"ldpc_sim_scalar": "cli_scalar.cpp", "ldpc_sim_circulant": "cli_circulant.cpp"}
-HEADERS = ["ar4ja_schedule.h", "device_math.h", "kernels.h", "runtime_internal.h"]
+HEADERS = ["ar4ja_schedule.h", "device_math.h", "kernels.h", "profile_r.h", "runtime_internal.h"]
 OBJ_DIR = os.path.join(PKG, "_build")
 # PyTorch extension (differentiable decoder): its own module, loaded into the
 # torch process so that its kernels run on torch's HIP runtime and streams
@@ -45,7 +45,7 @@ HIPCC_FLAGS = [
 # no operation makes a NaN), so fminf / fmaxf need no canonicalising of their
 # operands, which otherwise costs an instruction per leaf and registers.
 # Only device code: host code validates user floats with isfinite.
-KERNEL_SOURCES = ("decode_t.hip", "decode_r.hip", MULTI)
+KERNEL_SOURCES = ("decode_t.hip", "decode_r.hip", "decode_r2.hip", MULTI)
 KERNEL_FLAGS = ["-Xarch_device", "-fno-honor-nans"]
 
 

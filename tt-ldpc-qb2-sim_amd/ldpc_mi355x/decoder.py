@@ -84,6 +84,23 @@ def row_passes(rows, n, width=64):
     return out
 
 
+def row_passes_by_degree(rows, n, width=32):
+    """(pass of every row, kind of every pass) in libldpc's two-codeword
+    Profile R schedule: kind 3 / 6 = every row of the pass has that degree,
+    7 = rows of any degree (masked datapath)."""
+    lib = L.load()
+    rp = np.zeros(len(rows) + 1, dtype=np.int32)
+    rp[1:] = np.cumsum([len(r) for r in rows])
+    col = np.concatenate([np.asarray(r, dtype=np.int32) for r in rows]).astype(np.int32)
+    out = np.zeros(len(rows), dtype=np.int32)
+    kind = np.zeros(len(rows), dtype=np.uint8)
+    npass = lib.ldpc_row_passes_by_degree(n, len(rows), rp.ctypes.data, col.ctypes.data, width, out.ctypes.data,
+                                          kind.ctypes.data, len(kind))
+    if npass < 0:
+        raise L.LdpcError(npass, "invalid matrix")
+    return out, kind[:npass].copy()
+
+
 def generator(k_info: int, rate: str) -> np.ndarray:
     """Quasi-cyclic generator of an AR4JA code as uint32 words
     [12 parity blocks][k_info / z info blocks][z / 32] (ldpc_get_generator):

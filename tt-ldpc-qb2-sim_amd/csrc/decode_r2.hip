@@ -28,7 +28,8 @@
 //    argmin and the six signs.  Pass p = 4 g + j keeps them in P_j[g] and in
 //    bits 7 j .. 7 j + 6 of Mt[g] (five 16-dword vectors, 80 VGPRs, up to 64
 //    passes), read and written at the uniform index g through s_set_gpr_idx.
-//    LDS holds only the two codewords' LLRs.
+//    LDS holds only the two codewords' LLRs, each variable in the slot the
+//    host chose so that the lanes of a gather fall in distinct banks.
 //  * Each half converges on its own: after an iteration a half whose codeword
 //    passed the syndrome check, or ran out of iterations, is tallied and
 //    refilled from the work queue while the other half keeps decoding.
@@ -55,6 +56,7 @@ __device__ __forceinline__ void wave_fence() { __builtin_amdgcn_fence(__ATOMIC_S
 struct R2Args {
     const uint4 *table;      // (4 n_groups + 1) * 64 entries: degree, then six 16-bit LDS byte offsets
     const uint32_t *kinds;   // n_groups words; byte j of word g = kind of pass 4 g + j
+    const uint16_t *slot;    // LDS slot of every variable; transmitted ones below n_tx
     int n_groups, n, n_tx, max_iter, hs;
     float r_max, l_post_max, l_max;
 };
@@ -248,15 +250,17 @@ __device__ uint32_t syndrome2(const uint8_t *lds, const R2Args &a, int lane, uin
     return chk & ~bad;
 }
 
-// Channel LLRs (all-zero codeword) of codeword w into the half's LLRs, with
-// the clamp |L| <= l_max before rounding (ldpc_decoder_core.h:266-275): the
-// stream of decode_r.hip, generated by the half's 32 lanes.  Zeroes the
-// punctured variables and the zero slot at index n.
-__device__ void half_gen_channel(uint16_t *Lh, int hl, uint64_t seed, uint64_t w, float mu, float sigma, float l_max,
-                                 int n_tx, int n) {
+// Channel LLRs (all-zero codeword) of codeword w into one half's LLRs Lt,
+// with the clamp |L| <= l_max before rounding (ldpc_decoder_core.h:266-275):
+// the stream of decode_r.hip, generated by gs lanes (gl = 0 .. gs - 1);
+// variable i goes to LDS slot slot[i].  Zeroes the punctured variables'
+// slots (n_tx .. n - 1) and the zero slot n.
+__device__ void gen_channel_r2(uint16_t *Lt, const uint16_t *__restrict__ slot, int gl, int gs, uint64_t seed,
+                               uint64_t w, float mu, float sigma, float l_max, int n_tx, int n) {
     const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
     const int nblk = (n_tx + 3) / 4;
-    for (int blk = hl; blk < nblk; blk += 32) {
+    for (int blk = gl; blk < nblk; blk += gs) {
+        const uint2 sl = *(const uint2 *)(slot + 4 * blk);  // slots of variables 4 blk .. 4 blk + 3
         const u32x4 x = philox4x32_10((uint32_t)w, (uint32_t)(w >> 32), (uint32_t)blk, 0u, k0, k1);
         float z[4];
         box_muller(x.x, x.y, z[0], z[1]);
@@ -272,11 +276,12 @@ __device__ void half_gen_channel(uint16_t *Lh, int hl, uint64_t seed, uint64_t w
                     else if (l < -l_max)
                         l = -l_max;
                 }
-                Lh[i] = (uint16_t)(rne_bits(l) >> 16);
+                const uint32_t sw = j < 2 ? sl.x : sl.y;
+                Lt[(j & 1) ? (sw >> 16) : (sw & 0xFFFFu)] = (uint16_t)(rne_bits(l) >> 16);
             }
         }
     }
-    for (int i = n_tx + hl; i <= n; i += 32) Lh[i] = 0;
+    for (int i = n_tx + gl; i <= n; i += gs) Lt[i] = 0;
 }
 
 template <int MODE, bool NMS, bool CLAMP, bool MC>
@@ -310,15 +315,23 @@ __global__ void __launch_bounds__(64) k_r2(R2Args a, R2Work w) {
                 fill |= cw1 < w.n_cw ? 2u : 0u;
             }
             act = (act & ~need) | fill;
-            if ((fill >> h) & 1u) {
-                const uint32_t my = h ? cw1 : cw0;
+            if (fill) {
+                // both halves refilled: each half loads its own codeword; one
+                // half refilled: all 64 lanes load it
+                const bool both = fill == 3u;
+                const uint32_t th = both ? (uint32_t)h : fill >> 1;  // the half this lane loads
+                const int gl = both ? hl : lane, gs = both ? 32 : 64;
+                const uint32_t my = th ? cw1 : cw0;
+                uint16_t *Lt = (uint16_t *)(lds_r2 + th * a.hs);
                 if constexpr (MC) {
-                    half_gen_channel(Lh, hl, w.seed, w.first_cw + my, w.mu, w.sigma, a.l_max, a.n_tx, a.n);
+                    gen_channel_r2(Lt, a.slot, gl, gs, w.seed, w.first_cw + my, w.mu, w.sigma, a.l_max, a.n_tx, a.n);
                 } else {
                     const uint16_t *src = w.llr_in + (size_t)my * a.n;
-                    for (int i = hl; i < a.n; i += 32) Lh[i] = src[i];
-                    if (hl == 0) Lh[a.n] = 0;
+                    for (int i = gl; i < a.n; i += gs) Lt[a.slot[i]] = src[i];
+                    if (gl == 0) Lt[a.n] = 0;
                 }
+            }
+            if ((fill >> h) & 1u) {
                 P0 = 0;
                 P1 = 0;
                 P2 = 0;
@@ -358,7 +371,7 @@ __global__ void __launch_bounds__(64) k_r2(R2Args a, R2Work w) {
                 const uint32_t my = h ? cw1 : cw0;
                 const int it = ((conv >> h) & 1u) ? (h ? it1 : it0) : 0;  // 0: not converged
                 if constexpr (MC) {
-                    uint32_t b = 0;
+                    uint32_t b = 0;  // transmitted variables sit in slots 0 .. n_tx - 1
                     for (int i = hl; i < a.n_tx; i += 32) b += (uint32_t)(Lh[i] >> 15);
 #pragma unroll
                     for (int o = 16; o; o >>= 1) b += (uint32_t)__shfl_xor((int)b, o);
@@ -372,7 +385,7 @@ __global__ void __launch_bounds__(64) k_r2(R2Args a, R2Work w) {
                     }
                 } else {
                     uint16_t *dst = w.llr_out + (size_t)my * a.n;
-                    for (int i = hl; i < a.n; i += 32) dst[i] = Lh[i];
+                    for (int i = hl; i < a.n; i += 32) dst[i] = Lh[a.slot[i]];
                     if (hl == 0) {
                         w.iters[my] = it ? (uint32_t)it : (uint32_t)a.max_iter;
                         w.ok[my] = it ? 1 : 0;
@@ -394,8 +407,8 @@ __global__ void __launch_bounds__(64) k_r2(R2Args a, R2Work w) {
 }
 
 R2Args args_of(const RLaunch &L) {
-    return R2Args{(const uint4 *)L.table2, L.kinds2, L.n_groups, L.n,          L.n_tx,
-                  L.max_iter,            L.hs,     L.r_max,    L.l_post_max, L.l_max};
+    return R2Args{(const uint4 *)L.table2, L.kinds2, L.slot2,    L.n_groups, L.n,          L.n_tx,
+                  L.max_iter,            L.hs,     L.r_max, L.l_post_max, L.l_max};
 }
 
 template <int MODE, bool NMS, bool CLAMP, bool MC>

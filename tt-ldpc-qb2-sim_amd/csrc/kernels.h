@@ -74,6 +74,7 @@ struct RLaunch {
     int r2_mode = -1;                  // -1: not used; 0: passes of kind EXACT3 / EXACT6; 1: MASKED passes
     const void *table2 = nullptr;      // (4 n_groups + 1) * 64 entries of 16 B
     const uint32_t *kinds2 = nullptr;  // n_groups words, byte j of word g = kind of pass 4 g + j
+    const uint16_t *slot2 = nullptr;   // LDS slot of every variable (transmitted ones below n_tx), padded to 4
     int n_groups = 0;                  // ceil(passes / 4)
     int hs = 0;                        // bytes of LDS per half: the LLRs, a zero slot, padding
 };

@@ -60,6 +60,7 @@ struct ldpc_ctx {
     // two-codeword kernel)
     void *d_table = nullptr;
     uint32_t *d_kinds = nullptr;
+    uint16_t *d_slot = nullptr;
     ldpc::RLaunch rl{};
     DevBuf in, out, iters, ok, tails;
     int cus = 0;
@@ -290,6 +291,86 @@ int schedule_passes_by_degree(uint32_t n, uint32_t m, const int32_t *rp, const i
     return (int)kind.size();
 }
 
+// LDS slot of every variable for the two-codeword Profile R kernel.  Each
+// (pass, edge position) is one gather: a ds_read_u16 whose lanes read one
+// variable each (the zero slot n on idle lanes and past a row's degree).  The
+// LDS serves a half-wave's reads in one cycle when they fall in distinct
+// 4-byte banks (bank = (slot / 2) mod 32) and in k cycles when k share one.
+// The kernel reaches the variables only through the pass table, the channel
+// generator and the LLR copies, which all go through this map, so the slots
+// are free: a local search swaps the slots of two variables while that lowers
+// the sum over gathers of the squared reads per bank (simulated annealing,
+// fixed seed, so the map is deterministic).  Transmitted variables keep slots
+// below n_tx and punctured ones slots in [n_tx, n): the tally of the sign bits
+// of the transmitted variables still reads slots 0 .. n_tx - 1.
+std::vector<uint16_t> place_for_banks(uint32_t n, uint32_t n_tx, uint32_t m, const int32_t *rp, const int32_t *col,
+                                      const int32_t *pass, int np) {
+    constexpr uint32_t B = 32;
+    std::vector<std::vector<uint32_t>> rows_of(np);
+    for (uint32_t r = 0; r < m; ++r) rows_of[pass[r]].push_back(r);
+    std::vector<std::vector<uint32_t>> gathers;
+    for (int p = 0; p < np; ++p)
+        for (int d = 0; d < 6; ++d) {
+            std::vector<uint32_t> g;
+            bool zero = rows_of[p].size() < (size_t)ldpc::R2_PASS_ROWS;
+            for (uint32_t r : rows_of[p]) {
+                if (rp[r + 1] - rp[r] > d)
+                    g.push_back((uint32_t)col[rp[r] + d]);
+                else
+                    zero = true;
+            }
+            if (g.empty()) continue;
+            if (zero) g.push_back(n);
+            gathers.push_back(std::move(g));
+        }
+    std::vector<uint32_t> slot(n + 1);
+    for (uint32_t v = 0; v <= n; ++v) slot[v] = v;
+    std::vector<std::vector<uint32_t>> gof(n + 1);
+    for (uint32_t g = 0; g < gathers.size(); ++g)
+        for (uint32_t v : gathers[g]) gof[v].push_back(g);
+    std::vector<int32_t> load(gathers.size() * B, 0);
+    auto bank = [](uint32_t s) { return (s >> 1) & (B - 1); };
+    for (uint32_t g = 0; g < gathers.size(); ++g)
+        for (uint32_t v : gathers[g]) ++load[g * B + bank(slot[v])];
+    uint64_t x = 0x9E3779B97F4A7C15ull;
+    auto rnd = [&x]() {
+        x ^= x << 13;
+        x ^= x >> 7;
+        x ^= x << 17;
+        return x;
+    };
+    // move v's read from bank from to bank to in each of its gathers; returns the change of the objective
+    auto shift = [&](uint32_t v, uint32_t from, uint32_t to) {
+        int32_t d = 0;
+        for (uint32_t g : gof[v]) {
+            int32_t &lf = load[g * B + from], &lt = load[g * B + to];
+            d += 2 * (lt - lf) + 2;
+            --lf;
+            ++lt;
+        }
+        return d;
+    };
+    const uint64_t iters = std::min<uint64_t>(4000000, 400 * (uint64_t)n);
+    for (uint64_t it = 0; it < iters; ++it) {
+        const bool tx = (rnd() % n) < n_tx || n_tx == n;
+        const uint32_t lo = tx ? 0 : n_tx, span = tx ? n_tx : n - n_tx;
+        if (span < 2) continue;
+        const uint32_t a = lo + (uint32_t)(rnd() % span), b = lo + (uint32_t)(rnd() % span);
+        const uint32_t ba = bank(slot[a]), bb = bank(slot[b]);
+        if (ba == bb) continue;
+        const int32_t d = shift(a, ba, bb) + shift(b, bb, ba);
+        const double t = 2.0 * (1.0 - (double)it / (0.7 * (double)iters));
+        const bool take = d <= 0 || (t > 0.0 && (double)(rnd() >> 11) * 0x1.0p-53 < std::exp(-(double)d / t));
+        if (take) {
+            std::swap(slot[a], slot[b]);
+        } else {
+            shift(b, ba, bb);
+            shift(a, bb, ba);
+        }
+    }
+    return std::vector<uint16_t>(slot.begin(), slot.end() - 1);
+}
+
 int validate_rows(uint32_t n, uint32_t m, const int32_t *rp, const int32_t *col, std::string &why) {
     if (!rp || !col || n == 0 || m == 0) {
         why = "matrix: empty or null arrays";
@@ -340,7 +421,10 @@ int setup_profile_r(ldpc_ctx *c, const std::vector<int32_t> &rp, const std::vect
     // 64 KiB of LDS.  LDPC_R_ONE_CW=1 in the environment keeps the
     // one-codeword kernel (for comparisons).
     const char *one_cw = std::getenv("LDPC_R_ONE_CW");
-    const uint32_t hs = (2 * (n + 1) + 15) & ~15u;  // bytes of LDS per half: LLRs, zero slot at n
+    // bytes of LDS per half: the LLRs and the zero slot at n, padded to 128 mod
+    // 256 so that the second half's banks sit 32 words from the first's
+    uint32_t hs = (2 * (n + 1) + 15) & ~15u;
+    hs += (128u + 256u - hs % 256u) % 256u;
     if (maxd <= 6 && 2 * hs <= 65536 && !(one_cw && one_cw[0] == '1')) {
         std::vector<int32_t> pass(m);
         std::vector<uint8_t> kind;
@@ -353,6 +437,7 @@ int setup_profile_r(ldpc_ctx *c, const std::vector<int32_t> &rp, const std::vect
             // degree (0 on idle lanes); words 1..3: LDS byte offsets of the
             // row's variables, 16 bits each, the half's zero slot past the degree
             std::vector<uint32_t> table(n_entries * 4, 0u);
+            const std::vector<uint16_t> slot = place_for_banks(n, n - np, m, rp.data(), col.data(), pass.data(), np2);
             const uint32_t zero = 2 * n;
             for (size_t e = 0; e < n_entries; ++e) {
                 const uint32_t z = zero + (e % 64 >= 32 ? hs : 0);
@@ -366,7 +451,7 @@ int setup_profile_r(ldpc_ctx *c, const std::vector<int32_t> &rp, const std::vect
                     uint32_t *e = &table[((size_t)pass[r] * 64 + k + 32 * h) * 4];
                     e[0] = deg;
                     for (uint32_t d = 0; d < deg; ++d) {
-                        const uint32_t o = 2 * (uint32_t)col[rp[r] + d] + (h ? hs : 0);
+                        const uint32_t o = 2 * (uint32_t)slot[col[rp[r] + d]] + (h ? hs : 0);
                         uint32_t &wd = e[1 + d / 2];
                         wd = (d & 1) ? ((wd & 0xFFFFu) | (o << 16)) : ((wd & 0xFFFF0000u) | o);
                     }
@@ -378,9 +463,15 @@ int setup_profile_r(ldpc_ctx *c, const std::vector<int32_t> &rp, const std::vect
             HIP_TRY(c, hipMemcpy(c->d_table, table.data(), table.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
             HIP_TRY(c, hipMalloc(&c->d_kinds, kinds.size() * sizeof(uint32_t)));
             HIP_TRY(c, hipMemcpy(c->d_kinds, kinds.data(), kinds.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+            // slot map, padded to a multiple of 4 entries (the generator reads 4 at a time)
+            std::vector<uint16_t> slot4((n + 3) & ~3u, 0);
+            std::copy(slot.begin(), slot.end(), slot4.begin());
+            HIP_TRY(c, hipMalloc(&c->d_slot, slot4.size() * sizeof(uint16_t)));
+            HIP_TRY(c, hipMemcpy(c->d_slot, slot4.data(), slot4.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
             c->rl.r2_mode = kind[0] == ldpc::R2_PK_MASKED ? 1 : 0;
             c->rl.table2 = c->d_table;
             c->rl.kinds2 = c->d_kinds;
+            c->rl.slot2 = c->d_slot;
             c->rl.n_groups = n_groups;
             c->rl.hs = (int)hs;
             c->rl.n_pass = np2;
@@ -690,6 +781,7 @@ void ldpc_destroy(ldpc_ctx *c) {
         if (b->p) (void)hipFree(b->p);
     if (c->d_table) (void)hipFree(c->d_table);
     if (c->d_kinds) (void)hipFree(c->d_kinds);
+    if (c->d_slot) (void)hipFree(c->d_slot);
     if (c->d_cw_gen) (void)hipFree(c->d_cw_gen);
     if (c->d_alpha) (void)hipFree(c->d_alpha);
     if (c->d_cap_count) (void)hipFree(c->d_cap_count);

@@ -108,48 +108,62 @@ __device__ __forceinline__ void row2(uint8_t *lds, const uint4 &e, uint32_t &pai
         q[d] = __fsub_rn(u2f(lh[d] << 16), u2f(r));
         sign ^= (!MASKED || d < deg) ? f2u(q[d]) : 0u;
     }
-    int idx = 0;
-    float r_min, r_all;
+    // c[d]: edge d lowers the running minimum (strict <, so the first index
+    // wins ties); the argmin is the last such edge, or edge 0 when none does.
+    // The per-edge facts below are lane masks, combined by scalar ops.
+    bool c[D];
+    float r_min, r_all, min_mag;
+    float a[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) a[d] = absbits(q[d]);
     if constexpr (NMS) {
         float min1 = 999.0f, min2 = 999.0f;
 #pragma unroll
         for (int d = 0; d < D; ++d) {
-            const float a = absbits(q[d]);
-            const bool c1 = (!MASKED || d < deg) && a < min1;
-            const bool c2 = (!MASKED || d < deg) && !c1 && a < min2;
-            min2 = c1 ? min1 : (c2 ? a : min2);
-            min1 = c1 ? a : min1;
-            idx = c1 ? d : idx;
+            c[d] = (!MASKED || d < deg) && a[d] < min1;
+            const bool c2 = (!MASKED || d < deg) && !c[d] && a[d] < min2;
+            min2 = c[d] ? min1 : (c2 ? a[d] : min2);
+            min1 = c[d] ? a[d] : min1;
         }
+        min_mag = min1;
         r_min = __fmul_rn(0.75f, min2);
         r_all = __fmul_rn(0.75f, min1);
     } else {
-        float a[D];
-#pragma unroll
-        for (int d = 0; d < D; ++d) a[d] = absbits(q[d]);
-        float min_mag = 9999.0f;
+        min_mag = 9999.0f;
 #pragma unroll
         for (int d = 0; d < D; ++d) {
-            const bool c = (!MASKED || d < deg) && a[d] < min_mag;
-            min_mag = c ? a[d] : min_mag;
-            idx = c ? d : idx;
+            c[d] = (!MASKED || d < deg) && a[d] < min_mag;
+            min_mag = c[d] ? a[d] : min_mag;
         }
+    }
+    bool later[D];  // later[d]: some edge after d lowers the minimum, i.e. d < argmin
+    later[D - 1] = false;
+#pragma unroll
+    for (int d = D - 2; d >= 0; --d) later[d] = later[d + 1] || c[d + 1];
+    bool is_min[D];
+    is_min[0] = !later[0];
+#pragma unroll
+    for (int d = 1; d < D; ++d) is_min[d] = c[d] && !later[d];
+    uint32_t idx = 0;
+#pragma unroll
+    for (int d = 1; d < D; ++d) idx = is_min[d] ? (uint32_t)d : idx;
+    if constexpr (!NMS) {
         if constexpr (!MASKED) {
             // the non-minimum edges in index order (ldpc_decoder_core.h:180-190)
             float o[D - 1];
 #pragma unroll
-            for (int k = 0; k < D - 1; ++k) o[k] = k >= idx ? a[k + 1] : a[k];
+            for (int k = 0; k < D - 1; ++k) o[k] = later[k] ? a[k] : a[k + 1];
             float m = o[0];
 #pragma unroll
             for (int k = 1; k < D - 1; ++k) m = fold_r(m, o[k]);
             r_min = m;
         } else {
-            const int first = idx == 0 ? 1 : 0;
-            float m = deg > 1 ? (first ? a[1] : a[0]) : min_mag;
+            const bool first1 = is_min[0];  // the fold starts at edge 1 when edge 0 is the argmin
+            float m = deg > 1 ? (first1 ? a[1] : a[0]) : min_mag;
 #pragma unroll
             for (int d = 1; d < D; ++d) {
                 const float f = fold_r(m, a[d]);
-                m = (d > first && d < deg && d != idx) ? f : m;
+                m = ((d > 1 || !first1) && d < deg && !is_min[d]) ? f : m;
             }
             r_min = m;
         }
@@ -173,7 +187,7 @@ __device__ __forceinline__ void row2(uint8_t *lds, const uint4 &e, uint32_t &pai
     float ln[D];
 #pragma unroll
     for (int d = 0; d < D; ++d) {
-        ln[d] = __fadd_rn(q[d], u2f((d == idx ? rmin_b : rall_b) | s[d]));
+        ln[d] = __fadd_rn(q[d], u2f((is_min[d] ? rmin_b : rall_b) | s[d]));
         if (CLAMP && cfg.l_post_max > 0.0f) ln[d] = __builtin_amdgcn_fmed3f(ln[d], -cfg.l_post_max, cfg.l_post_max);
     }
     uint32_t lw[(D + 1) / 2];
@@ -285,7 +299,7 @@ __device__ void gen_channel_r2(uint16_t *Lt, const uint16_t *__restrict__ slot, 
 }
 
 template <int MODE, bool NMS, bool CLAMP, bool MC>
-__global__ void __launch_bounds__(64) k_r2(R2Args a, R2Work w) {
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_r2(R2Args a, R2Work w) {
     extern __shared__ __attribute__((aligned(16))) uint8_t lds_r2[];
     const int lane = threadIdx.x;
     const int h = lane >> 5, hl = lane & 31;

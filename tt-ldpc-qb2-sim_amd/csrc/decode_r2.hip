@@ -79,7 +79,7 @@ struct R2Work {
 
 // One row on one lane.  e: the lane's table entry (degree in bits 0..3, 0 on
 // idle lanes; LDS byte offsets of the row's variables in 16-bit fields, the
-// half's zero slot past the degree and on idle lanes).  pair / meta: the
+// half's slot 0 past the degree and on idle lanes: read, never used).  pair / meta: the
 // row's compressed messages, updated in place:
 //   pair: bits 0..14 bf16(r_min), bit 15 s_0, bits 16..30 bf16(r_all), bit 31 s_1
 //   meta: bits 0..2 argmin, bit d + 1 s_d for d >= 2
@@ -203,10 +203,11 @@ __device__ __forceinline__ void row2(uint8_t *lds, const uint4 &e, uint32_t &pai
     }
 }
 
-// Parity of a row on the raw sign bits.  Slots past the degree and idle
-// lanes read the half's zero slot, so no masking is needed.
-template <int D>
+// Parity of a row on the raw sign bits; 0 on idle lanes.  MASKED: the
+// slots past the degree are left out.
+template <int D, bool MASKED>
 __device__ __forceinline__ uint32_t parity2(const uint8_t *lds, const uint4 &e) {
+    const int deg = (int)(e.x & 15u);
     const uint32_t wv[3] = {e.y, e.z, e.w};
     uint32_t h[D];
 #pragma unroll
@@ -215,8 +216,8 @@ __device__ __forceinline__ uint32_t parity2(const uint8_t *lds, const uint4 &e) 
     __builtin_amdgcn_sched_barrier(0);
     uint32_t par = 0;
 #pragma unroll
-    for (int d = 0; d < D; ++d) par ^= h[d];
-    return (par >> 15) & 1u;
+    for (int d = 0; d < D; ++d) par ^= (!MASKED || d < deg) ? h[d] : 0u;
+    return deg ? (par >> 15) & 1u : 0u;
 }
 
 // MODE 0: every pass is PK_EXACT3 or PK_EXACT6 (or PK_NONE past the last);
@@ -255,7 +256,8 @@ __device__ uint32_t syndrome2(const uint8_t *lds, const R2Args &a, int lane, uin
             nx = a.table[(size_t)(4 * g + j + 1) * 64 + lane];
             const uint32_t kind = (kw >> (8 * j)) & 0xFFu;
             if (kind == R2_PK_NONE) continue;
-            const uint32_t par = kind == R2_PK_EXACT3 ? parity2<3>(lds, e) : parity2<6>(lds, e);
+            const uint32_t par = MODE == 1 ? parity2<6, true>(lds, e)
+                                 : kind == R2_PK_EXACT3 ? parity2<3, false>(lds, e) : parity2<6, false>(lds, e);
             const uint64_t b = __ballot(par);
             bad |= ((uint32_t)b ? 1u : 0u) | ((uint32_t)(b >> 32) ? 2u : 0u);
             if ((bad & chk) == chk) return 0;
@@ -268,7 +270,7 @@ __device__ uint32_t syndrome2(const uint8_t *lds, const R2Args &a, int lane, uin
 // with the clamp |L| <= l_max before rounding (ldpc_decoder_core.h:266-275):
 // the stream of decode_r.hip, generated by gs lanes (gl = 0 .. gs - 1);
 // variable i goes to LDS slot slot[i].  Zeroes the punctured variables'
-// slots (n_tx .. n - 1) and the zero slot n.
+// slots (n_tx .. n - 1).
 __device__ void gen_channel_r2(uint16_t *Lt, const uint16_t *__restrict__ slot, int gl, int gs, uint64_t seed,
                                uint64_t w, float mu, float sigma, float l_max, int n_tx, int n) {
     const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
@@ -295,7 +297,7 @@ __device__ void gen_channel_r2(uint16_t *Lt, const uint16_t *__restrict__ slot, 
             }
         }
     }
-    for (int i = n_tx + gl; i <= n; i += gs) Lt[i] = 0;
+    for (int i = n_tx + gl; i < n; i += gs) Lt[i] = 0;
 }
 
 template <int MODE, bool NMS, bool CLAMP, bool MC>
@@ -342,7 +344,6 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
                 } else {
                     const uint16_t *src = w.llr_in + (size_t)my * a.n;
                     for (int i = gl; i < a.n; i += gs) Lt[a.slot[i]] = src[i];
-                    if (gl == 0) Lt[a.n] = 0;
                 }
             }
             if ((fill >> h) & 1u) {

@@ -76,7 +76,7 @@ struct RLaunch {
     const uint32_t *kinds2 = nullptr;  // n_groups words, byte j of word g = kind of pass 4 g + j
     const uint16_t *slot2 = nullptr;   // LDS slot of every variable (transmitted ones below n_tx), padded to 4
     int n_groups = 0;                  // ceil(passes / 4)
-    int hs = 0;                        // bytes of LDS per half: the LLRs, a zero slot, padding
+    int hs = 0;                        // bytes of LDS per half (the LLRs, rounded up to 16)
 };
 int r_waves_per_wg();
 size_t r_lds_bytes(int n, int n_edges);

@@ -293,7 +293,7 @@ int schedule_passes_by_degree(uint32_t n, uint32_t m, const int32_t *rp, const i
 
 // LDS slot of every variable for the two-codeword Profile R kernel.  Each
 // (pass, edge position) is one gather: a ds_read_u16 whose lanes read one
-// variable each (the zero slot n on idle lanes and past a row's degree).  The
+// variable each (slot 0 on idle lanes and past a row's degree).  The
 // LDS serves a half-wave's reads in one cycle when they fall in distinct
 // 4-byte banks (bank = (slot / 2) mod 32) and in k cycles when k share one.
 // The kernel reaches the variables only through the pass table, the channel
@@ -324,7 +324,8 @@ std::vector<uint16_t> place_for_banks(uint32_t n, uint32_t n_tx, uint32_t m, con
             gathers.push_back(std::move(g));
         }
     std::vector<uint32_t> slot(n + 1);
-    for (uint32_t v = 0; v <= n; ++v) slot[v] = v;
+    for (uint32_t v = 0; v < n; ++v) slot[v] = v;
+    slot[n] = 0;  // the reads of idle lanes and of slots past a row's degree
     std::vector<std::vector<uint32_t>> gof(n + 1);
     for (uint32_t g = 0; g < gathers.size(); ++g)
         for (uint32_t v : gathers[g]) gof[v].push_back(g);
@@ -421,10 +422,9 @@ int setup_profile_r(ldpc_ctx *c, const std::vector<int32_t> &rp, const std::vect
     // 64 KiB of LDS.  LDPC_R_ONE_CW=1 in the environment keeps the
     // one-codeword kernel (for comparisons).
     const char *one_cw = std::getenv("LDPC_R_ONE_CW");
-    // bytes of LDS per half: the LLRs and the zero slot at n, padded to 128 mod
-    // 256 so that the second half's banks sit 32 words from the first's
-    uint32_t hs = (2 * (n + 1) + 15) & ~15u;
-    hs += (128u + 256u - hs % 256u) % 256u;
+    // bytes of LDS per half: the LLRs (2560 variables: 5 KiB, 16 workgroups
+    // of one wave per CU)
+    const uint32_t hs = (2 * n + 15) & ~15u;
     if (maxd <= 6 && 2 * hs <= 65536 && !(one_cw && one_cw[0] == '1')) {
         std::vector<int32_t> pass(m);
         std::vector<uint8_t> kind;
@@ -435,10 +435,10 @@ int setup_profile_r(ldpc_ctx *c, const std::vector<int32_t> &rp, const std::vect
             // entry (p, lane): lane k < 32 runs the k-th row of pass p on the
             // first codeword, lane k + 32 on the second (offsets + hs).  Word 0:
             // degree (0 on idle lanes); words 1..3: LDS byte offsets of the
-            // row's variables, 16 bits each, the half's zero slot past the degree
+            // row's variables, 16 bits each, the half's slot 0 past the degree
             std::vector<uint32_t> table(n_entries * 4, 0u);
             const std::vector<uint16_t> slot = place_for_banks(n, n - np, m, rp.data(), col.data(), pass.data(), np2);
-            const uint32_t zero = 2 * n;
+            const uint32_t zero = 0;
             for (size_t e = 0; e < n_entries; ++e) {
                 const uint32_t z = zero + (e % 64 >= 32 ? hs : 0);
                 for (int w = 1; w < 4; ++w) table[e * 4 + w] = z | (z << 16);

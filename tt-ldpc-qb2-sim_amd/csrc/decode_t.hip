@@ -363,35 +363,15 @@ __device__ __forceinline__ void wave_cw_errors(const uint16_t *Ls, const uint32_
     mism = any != 0;
 }
 
-// Work split of a launch: codewords [0, n_static) are dealt round-robin to the
-// waves (no atomics); the rest are handed out one at a time from a device
-// queue so that waves that drew easy codewords absorb the imbalance of the
-// tail.  n_static is 7/8 of the launch rounded down to whole rounds.
-struct WorkSplit {
-    uint32_t n_cw, n_static, n_waves, wave;
-    uint32_t k;  // next static round of this wave
-};
-
-__device__ __forceinline__ WorkSplit work_split(uint32_t n_cw) {
-    WorkSplit w;
-    w.n_cw = n_cw;
-    w.n_waves = gridDim.x * WAVES_PER_WG;
-    w.wave = blockIdx.x * WAVES_PER_WG + (threadIdx.x >> 6);
-    w.n_static = (uint32_t)(((uint64_t)n_cw * 7u / 8u) / w.n_waves * w.n_waves);
-    w.k = 0;
-    return w;
-}
-
-__device__ __forceinline__ uint32_t next_codeword(WorkSplit &w, uint32_t *queue, int lane) {
-    const uint32_t s = w.wave + w.k * w.n_waves;
-    if (s < w.n_static) {
-        ++w.k;
-        return s;
-    }
+// Work distribution: every codeword of a launch comes from a device queue,
+// one atomic per codeword.  Waves that drew codewords needing few iterations
+// (or that run on a faster CU) take more of them.  A round-robin static
+// share of 7/8 of the launch ran 13% slower on the r1/2 K=1024 point: the
+// per-wave work it fixes in advance cannot follow the wave's real speed.
+__device__ __forceinline__ uint32_t next_codeword(uint32_t *queue, int lane) {
     uint32_t v = 0;
     if (lane == 0) v = atomicAdd(queue, 1u);
-    v = __builtin_amdgcn_readfirstlane(v);
-    return v >= w.n_cw - w.n_static ? w.n_cw : w.n_static + v;
+    return __builtin_amdgcn_readfirstlane(v);
 }
 
 __global__ void __launch_bounds__(64 * WAVES_PER_WG)
@@ -400,9 +380,8 @@ __global__ void __launch_bounds__(64 * WAVES_PER_WG)
     __shared__ __attribute__((aligned(16))) uint16_t smem[WAVES_PER_WG][NV];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     uint16_t *Ls = smem[wv];
-    WorkSplit ws = work_split(n_cw);
     for (;;) {
-        const uint32_t cw = next_codeword(ws, queue, lane);
+        const uint32_t cw = next_codeword(queue, lane);
         if (cw >= n_cw) break;
         const uint4 *src = reinterpret_cast<const uint4 *>(llr_in + (size_t)cw * NV);
 #pragma unroll
@@ -444,9 +423,8 @@ __device__ __forceinline__ void monte_carlo(uint64_t seed, uint64_t first_cw, ui
     uint16_t *Ls = smem[wv];
     uint32_t *ub = reinterpret_cast<uint32_t *>(Ls + NV);
     uint64_t blocks = 0, fe = 0, be = 0, its = 0;
-    WorkSplit ws = work_split(n_cw);
     for (;;) {
-        const uint32_t cw = next_codeword(ws, queue, lane);
+        const uint32_t cw = next_codeword(queue, lane);
         if (cw >= n_cw) break;
         wave_gen_channel<RANDOM>(Ls, ub, lane, seed, first_cw + cw, mu, sigma, gen);
         wave_lds_fence();

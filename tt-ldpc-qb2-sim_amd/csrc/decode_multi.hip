@@ -115,7 +115,9 @@ struct Cfg {
     // Waves per SIMD the compiler must fit.  Rate 4/5 in wave mode with one
     // check per thread (K = 1024 r4/5) compiles to 171 VGPRs, 2 waves per
     // SIMD; held to 3 waves it fits in 167 with no spill.
-    static constexpr int MIN_WAVES = (LDPC_RID == 2 && WAVE && C == 1) ? 3 : 1;
+    static constexpr bool LDS_R45B = LDPC_RID == 2 && C == 1 && !WAVE && THREADS == 512;
+    static constexpr bool RECOMPUTE = LDS_R45B;
+    static constexpr int MIN_WAVES = (LDPC_RID == 2 && WAVE && C == 1) ? 3 : LDS_R45B ? 4 : 1;
     static constexpr bool LDS_R12 = !WAVE && THREADS == 1024 && C == 2;
     static constexpr bool LDS_R45 = LDPC_RID == 2 && C == 1 && !WAVE && THREADS <= 256;
     static constexpr int deg18_before(int li) {
@@ -125,7 +127,8 @@ struct Cfg {
     }
     static constexpr bool in_lds(int li) {
         return (LDS_R12 && S.layer[li].deg == 3) ||
-               (LDS_R45 && (S.layer[li].deg == 3 || (S.layer[li].deg == 18 && deg18_before(li) < 2)));
+               (LDS_R45 && (S.layer[li].deg == 3 || (S.layer[li].deg == 18 && deg18_before(li) < 2))) ||
+               (LDS_R45B && (S.layer[li].deg == 3 || (S.layer[li].deg == 18 && deg18_before(li) < 1)));
     }
     static constexpr int r_off(int li) {  // offset in the VGPR array
         int o = 0;
@@ -224,13 +227,22 @@ __device__ __forceinline__ void run_layer(uint16_t *Ls, uint32_t (&R)[CF::NR], u
         const uint32_t s1 = (sp ^ (sp >> 16)) & 0x8000u, s2 = s1 | (s1 << 16);
         float r[D];
         tree<D>(m, beta, r);
+        // Wide checks recompute their addresses for the stores instead of
+        // keeping 2 DW of them live across the tree.
+        int tw = t;
+        if constexpr (CF::RECOMPUTE && D >= 10) asm volatile("" : "+v"(tw));
 #pragma unroll
         for (int w = 0; w < DW; ++w) {
             const int d0 = 2 * w, d1 = 2 * w + 1;
             const uint32_t rn = cvt_pk_bf16(r[d0], d1 < D ? r[d1 < D ? d1 : 0] : 0.0f) | ((s2 ^ qw[w]) & 0x80008000u);
             const uint32_t ln = cvt_pk_bf16(__fadd_rn(q[d0], lo_f(rn)), __fadd_rn(q[d1], hi_f(rn)));
-            Ls[ia[d0]] = (uint16_t)ln;
-            if (d1 < D) Ls[ia[d1]] = (uint16_t)(ln >> 16);
+            const int w0 = CF::RECOMPUTE && D >= 10 ? Ly.slot[d0].base + ((tw + Ly.slot[d0].shift) & (Z - 1)) : ia[d0];
+            Ls[w0] = (uint16_t)ln;
+            if (d1 < D) {
+                const int w1 =
+                    CF::RECOMPUTE && D >= 10 ? Ly.slot[d1].base + ((tw + Ly.slot[d1].shift) & (Z - 1)) : ia[d1];
+                Ls[w1] = (uint16_t)(ln >> 16);
+            }
             if constexpr (CF::in_lds(LI))
                 Rl[(RL0 + w) * CF::THREADS] = rn;
             else
@@ -479,7 +491,8 @@ k_multi_wave(MultiParams P) {
 // Group mode
 // ---------------------------------------------------------------------------
 template <class CF>
-__global__ void __launch_bounds__(CF::THREADS) k_multi_group(MultiParams P) {
+__global__ void __launch_bounds__(CF::THREADS) __attribute__((amdgpu_waves_per_eu(CF::MIN_WAVES)))
+k_multi_group(MultiParams P) {
     __shared__ __attribute__((aligned(16))) uint16_t Ls[CF::N];
     __shared__ uint32_t Rl_s[CF::NRL > 0 ? CF::NRL * CF::THREADS : 1];
     __shared__ uint32_t hist_s[HIST_LDS];

@@ -134,6 +134,31 @@ def test_two_codeword_kernel_on_random_matrices(oracle, mix):
         np.testing.assert_array_equal(r.hist, o["hist"])
 
 
+@pytest.mark.parametrize("max_iter", [1, 2, 3])
+def test_two_codeword_kernel_iteration_caps(chinn, oracle, max_iter):
+    """Caps at and below the third iteration, where the syndrome check starts:
+    one codeword (one half of the wave idle), three codewords (a refill), and
+    a Monte Carlo point, against the oracle."""
+    M, N, rows = chinn
+    mu, sg = channel_params(2.5)
+    llr = oracle.gen_channel_clamped(N, 2048, 0x77, 0, 3, mu, sg)
+    for nms in (False, True):
+        _case(f"two-codeword kernel max_iter={max_iter} nms={nms}")
+        with _engine(chinn, profile="R_NMS" if nms else "R", max_iter=max_iter) as eng:
+            for k in (1, 3):
+                out, iters, ok = eng.decode_llrs(llr[:k])
+                o_out, o_it, o_ok = oracle.decode_r(N, rows, llr[:k], max_iter, nms)
+                np.testing.assert_array_equal(out, o_out)
+                np.testing.assert_array_equal(iters, o_it)
+                np.testing.assert_array_equal(ok, o_ok)
+            r = eng.run_point(2.5, 257, seed=4, first_cw=1, chunk_cw=257)
+            emu, esg = eng.channel_params(2.5)
+        o = oracle.run_point_r(N, rows, 2048, 4, 1, 257, emu, esg, np.float32(0.0), max_iter, nms, 0.0, 0.0)
+        assert (r.blocks, r.frame_errors, r.bit_errors, r.iter_sum) == \
+            (o["blocks"], o["frame_errors"], o["bit_errors"], o["iter_sum"])
+        np.testing.assert_array_equal(r.hist, o["hist"])
+
+
 def test_one_codeword_kernel_still_exact(chinn, oracle, monkeypatch):
     """LDPC_R_ONE_CW=1 keeps the one-codeword kernel (messages in LDS); it stays
     bit-exact on the shipped matrix, and a matrix whose dependency chain needs

@@ -74,7 +74,7 @@ def test_profile_r_and_d_capture(oracle):
 
 def test_profile_d_k16384_capture():
     """The K=16384 Profile D kernel records the ids of its failing codewords:
-    over 320 codewords (static split and queue) they equal those that
+    over 320 codewords (more than one per workgroup) they equal those that
     decode_llrs leaves unconverged on the same channel (decode arithmetic the
     Profile D tests pin to the oracle)."""
     import dmin_oracle as DO

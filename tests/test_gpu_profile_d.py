@@ -82,15 +82,15 @@ def test_monte_carlo_k16384_matches_oracle(oracle):
     np.testing.assert_array_equal(got.hist, exp["hist"])
 
 
-def test_monte_carlo_k16384_static_split_agrees_with_decode():
-    """320 codewords over 256 workgroups: 256 from the static split, 64 from
-    the queue.  The Monte Carlo counters and histogram equal the tallies of
+def test_monte_carlo_k16384_queue_agrees_with_decode():
+    """320 codewords over 256 workgroups, each taken from the device queue
+    (some workgroups take two).  The Monte Carlo counters and histogram equal the tallies of
     decoding the same channel (gen_channel + decode_llrs, whose arithmetic the
     decode test pins to the oracle)."""
     code = codes.ar4ja(16384, "1/2")
     a, b = _params_k16384(code)
     n, max_iter = 320, 30
-    print("Profile D K=16384 r1/2: static split, 320 codewords", flush=True)
+    print("Profile D K=16384 r1/2: queue, 320 codewords", flush=True)
     with decoder.Engine(16384, "1/2", max_iter=max_iter, profile="D") as eng:
         eng.set_dmin_params(a, b, True)
         got = eng.run_point(1.1, n, seed=29, first_cw=0, chunk_cw=n)

@@ -305,15 +305,22 @@ __device__ __forceinline__ uint32_t syndrome_pass(const uint16_t *Ls, int t, uin
     }
 }
 
-// Group mode: does any check of this wave's threads fail?
+// Group mode: does any check of the codeword fail?  The waves share one LDS
+// flag: a wave that finds an unsatisfied check among its threads' checks
+// sets it, and a wave stops scanning its layers as soon as it sees it set.
+// Near convergence only a few checks fail and most waves would otherwise
+// scan all 12 layers before the barrier; the answer is the same either way
+// (the flag is set if and only if some check fails).
 template <class CF, int LI>
-__device__ __forceinline__ bool wave_fails_from(const uint16_t *Ls, int t) {
-    if constexpr (LI == NLAYERS) {
-        return false;
-    } else {
+__device__ __forceinline__ void scan_fails_from(const uint16_t *Ls, int t, volatile uint32_t *fail) {
+    if constexpr (LI < NLAYERS) {
         asm volatile("" ::: "memory");
-        if (__ballot(layer_fails<CF, LI>(Ls, t)) != 0) return true;
-        return wave_fails_from<CF, LI + 1>(Ls, t);
+        if (__builtin_amdgcn_readfirstlane(*fail)) return;
+        if (__ballot(layer_fails<CF, LI>(Ls, t)) != 0) {
+            if ((t & 63) == 0) *fail = 1u;
+            return;
+        }
+        scan_fails_from<CF, LI + 1>(Ls, t, fail);
     }
 }
 
@@ -496,10 +503,12 @@ k_multi_group(MultiParams P) {
     __shared__ __attribute__((aligned(16))) uint16_t Ls[CF::N];
     __shared__ uint32_t Rl_s[CF::NRL > 0 ? CF::NRL * CF::THREADS : 1];
     __shared__ uint32_t hist_s[HIST_LDS];
-    __shared__ uint32_t flag_s[CF::WPW];
+    // fail_s[it & 1]: some check is unsatisfied after iteration it (two
+    // flags, so that the one being set is never the one being read or cleared)
+    __shared__ uint32_t fail_s[2];
     __shared__ uint32_t cw_s, be_s;
     const Run run = load_run(P);
-    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    const int t = threadIdx.x;
     uint32_t *Rl = Rl_s + t;
     for (int i = t; i < HIST_LDS; i += CF::THREADS) hist_s[i] = 0u;
     unsigned long long blocks = 0, fe = 0, be = 0, its = 0;
@@ -514,7 +523,10 @@ k_multi_group(MultiParams P) {
             copy_in<CF>(Ls, P.llr_in + (size_t)cw * CF::N, t);
         else
             gen_channel<CF>(Ls, t, run.seed, run.first_cw + cw, run.mu, run.sigma);
-        if (t == 0) be_s = 0u;
+        if (t == 0) {
+            be_s = 0u;
+            fail_s[0] = fail_s[1] = 0u;
+        }
 #pragma unroll
         for (int i = 0; i < CF::NR; ++i) R[i] = 0u;
 #pragma unroll
@@ -525,12 +537,12 @@ k_multi_group(MultiParams P) {
         while (it < P.max_iter) {
             run_iteration<CF>(Ls, R, Rl, t, P.beta, P.beta3, std::make_integer_sequence<int, NLAYERS>{});
             ++it;
-            const bool f = wave_fails_from<CF, 0>(Ls, t);
-            if (lane == 0) flag_s[wv] = f ? 1u : 0u;
+            scan_fails_from<CF, 0>(Ls, t, fail_s + (it & 1));
             __syncthreads();
-            uint32_t any = 0;
-#pragma unroll
-            for (int w = 0; w < CF::WPW; ++w) any |= flag_s[w];
+            const uint32_t any = fail_s[it & 1];
+            // every wave read the other flag after the previous iteration's
+            // barrier, before this iteration's layer barriers
+            if (t == 0) fail_s[(it + 1) & 1] = 0u;
             if (!any) {
                 ok = true;
                 break;
@@ -699,14 +711,17 @@ __device__ __forceinline__ uint32_t syndrome_pass_d(const float *Ls, int t, uint
     }
 }
 
+// The shared-flag scan of scan_fails_from, on fp32 LLRs.
 template <class CF, int LI>
-__device__ __forceinline__ bool wave_fails_from_d(const float *Ls, int t) {
-    if constexpr (LI == NLAYERS) {
-        return false;
-    } else {
+__device__ __forceinline__ void scan_fails_from_d(const float *Ls, int t, volatile uint32_t *fail) {
+    if constexpr (LI < NLAYERS) {
         asm volatile("" ::: "memory");
-        if (__ballot(layer_fails_d<CF, LI>(Ls, t)) != 0) return true;
-        return wave_fails_from_d<CF, LI + 1>(Ls, t);
+        if (__builtin_amdgcn_readfirstlane(*fail)) return;
+        if (__ballot(layer_fails_d<CF, LI>(Ls, t)) != 0) {
+            if ((t & 63) == 0) *fail = 1u;
+            return;
+        }
+        scan_fails_from_d<CF, LI + 1>(Ls, t, fail);
     }
 }
 
@@ -829,10 +844,12 @@ template <class CF>
 __global__ void __launch_bounds__(CF::THREADS) k_d_group(MultiParams P) {
     __shared__ __attribute__((aligned(16))) float Ls[CF::N];
     __shared__ uint32_t hist_s[HIST_LDS];
-    __shared__ uint32_t flag_s[CF::WPW];
+    // fail_s[it & 1]: some check is unsatisfied after iteration it (two
+    // flags, so that the one being set is never the one being read or cleared)
+    __shared__ uint32_t fail_s[2];
     __shared__ uint32_t cw_s, be_s;
     const Run run = load_run(P);
-    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    const int t = threadIdx.x;
     for (int i = t; i < HIST_LDS; i += CF::THREADS) hist_s[i] = 0u;
     const bool corr = P.d_corr != 0;
     const float rcap = P.d_rmax > 0.0f ? P.d_rmax : __builtin_inff();
@@ -848,7 +865,10 @@ __global__ void __launch_bounds__(CF::THREADS) k_d_group(MultiParams P) {
             copy_in_d<CF>(Ls, P.llr_in + (size_t)cw * CF::N, t);
         else
             gen_channel_d<CF>(Ls, t, run.seed, run.first_cw + cw, run.mu, run.sigma);
-        if (t == 0) be_s = 0u;
+        if (t == 0) {
+            be_s = 0u;
+            fail_s[0] = fail_s[1] = 0u;
+        }
 #pragma unroll
         for (int i = 0; i < CF::NR; ++i) R[i] = 0.0f;
         __syncthreads();
@@ -859,12 +879,10 @@ __global__ void __launch_bounds__(CF::THREADS) k_d_group(MultiParams P) {
             run_iteration_d<CF>(Ls, R, t, P.d_alpha + row * CF::NR, P.d_beta + row * CF::NR, corr, rcap,
                                 std::make_integer_sequence<int, NLAYERS>{});
             ++it;
-            const bool f = wave_fails_from_d<CF, 0>(Ls, t);
-            if (lane == 0) flag_s[wv] = f ? 1u : 0u;
+            scan_fails_from_d<CF, 0>(Ls, t, fail_s + (it & 1));
             __syncthreads();
-            uint32_t any = 0;
-#pragma unroll
-            for (int w = 0; w < CF::WPW; ++w) any |= flag_s[w];
+            const uint32_t any = fail_s[it & 1];
+            if (t == 0) fail_s[(it + 1) & 1] = 0u;  // read by every wave before this iteration's layers
             if (!any) {
                 ok = true;
                 break;

@@ -76,6 +76,19 @@ def ops_per_check(d: int) -> int:
     return 18 * d - 1 + 11 * n_amin
 
 
+def ops_per_row_r(d: int) -> int:
+    """VALU lane-ops of one Profile R row update of degree d, counted on
+    engines/scalar_5w/ldpc_decoder_core.h with the rules of ops_per_check (bf16
+    unpacks and abs free, RNE 4, a compare or a select 1).  Per edge 23: q 1
+    (:432), sign parity 2 (:435), running min and argmin 3 (:437-440),
+    message select 2 (:486), message sign 4 (:488-490), L' 1 (:491), RNE of L'
+    and of R' 8 (:496-497), two-codeword pack 1 (:518-519), syndrome parity 1.
+    Per fold step 13, d - 1 of them (:176-201: |diff| 1, min 1, correction 9 --
+    two compares, two lines of 2 and 3 ops, two selects --, subtract 1, clamp
+    at 0 1).  d = 6 gives 203, d = 3 gives 95."""
+    return 23 * d + 13 * (d - 1)
+
+
 def ops_per_cwi(k: int, rate: str) -> int:
     """Lane-ops of one codeword-iteration: M checks each of degree 3, d, d."""
     m = {(1024, "1/2"): 512, (1024, "2/3"): 256, (1024, "4/5"): 128}[(1024, rate)] * (k // 1024)
@@ -298,9 +311,9 @@ def bench_scalar(args) -> None:
     wall = time.perf_counter() - t0
     value = acc.blocks * N_TX / wall / 1e9
     per_launch_s = acc.kernel_s / acc.launches
-    # Profile R per edge: q 1, |q| and sign 2, min/argmin 2, fold step 9, output select/sign 3, two RNE 8,
-    # L' add 1 -> about 26 lane-ops; 7680 edges per codeword-iteration
-    ops = 26 * 7680 * acc.iter_sum / acc.launches
+    # lane-ops per codeword-iteration: 256,512 for the .chinn.out (1024 rows of degree 6, 512 of degree 3)
+    ops_cwi = sum(ops_per_row_r(len(r)) for r in rows)
+    ops = ops_cwi * acc.iter_sum / acc.launches
     cpu = None
     if not args.no_cpu:
         try:
@@ -320,7 +333,11 @@ def bench_scalar(args) -> None:
                    "max_iter": MAX_ITER, "codewords_per_step": CW_PER_STEP},
         "roofline": {"bound": "valu", "achieved": ops / per_launch_s / 1e12, "peak": PEAK_VALU_TOPS,
                      "unit": "Tlane-op/s", "frac": ops / per_launch_s / 1e12 / PEAK_VALU_TOPS, "traffic": None,
-                     "kernel": "k_r2", "kernel_ms_per_launch": per_launch_s * 1e3},
+                     "kernel": "k_r2", "kernel_ms_per_launch": per_launch_s * 1e3,
+                     "algorithmic_ops_per_cwi": ops_cwi,
+                     # the earlier flat estimate of 26 lane-ops per edge (one fold step per edge)
+                     "frac_at_26_per_edge": 26 * 7680 * acc.iter_sum / acc.launches / per_launch_s / 1e12
+                     / PEAK_VALU_TOPS},
         "cpu_baseline": cpu,
         "fer": acc.fer, "fer_ci95": [fe_lo / acc.blocks, fe_hi / acc.blocks], "frame_errors": acc.frame_errors,
         "ber": acc.bit_errors / (acc.blocks * N_TX), "mean_iterations": acc.mean_iterations, "blocks": acc.blocks,

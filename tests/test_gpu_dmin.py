@@ -152,6 +152,28 @@ def test_checkpoint_interval_changes_nothing():
             np.testing.assert_array_equal(x, y)
 
 
+def test_training_draws_mined_codewords():
+    """hard_frac = 1 with a one-codeword hard set: every row of the batch is
+    that codeword, so the first step's loss (taken before the update) is its
+    own BCE."""
+    D = dmin()
+    torch.manual_seed(0)
+    model = D.DMinSum(1024, "1/2", T=6, ckpt_every=12)
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(9)
+    one = D.channel_llr(model.code, torch.full((1,), 1.2, device="cuda"), gen)
+    with torch.no_grad():
+        own = float(D.soft_bce(model(one), model.n_tx))
+    log = D.train(model, steps=2, batch=2048, micro_batch=1024, lr=1e-3, ebn0_range=(2.0, 2.0), seed=3,
+                  log_every=1, hard=one, hard_frac=1.0)
+    assert abs(log[0].loss - own) <= 1e-5 * abs(own), (log[0].loss, own)
+    # and without it the loss is the channel's at 2.0 dB, far from this 1.2 dB codeword's
+    model2 = D.DMinSum(1024, "1/2", T=6, ckpt_every=12)
+    log2 = D.train(model2, steps=1, batch=2048, micro_batch=1024, lr=1e-3, ebn0_range=(2.0, 2.0), seed=3,
+                   log_every=1)
+    assert abs(log2[0].loss - own) > 1e-3 * abs(own)
+
+
 def test_training_steps_run():
     D = dmin()
     torch.manual_seed(0)

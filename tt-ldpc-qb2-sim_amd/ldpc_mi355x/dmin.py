@@ -144,11 +144,16 @@ class TrainLog:
 
 def train(model: DMinSum, steps: int, batch: int = 65536, micro_batch: int = 16384, lr: float = 1e-3,
           ebn0_range=(1.0, 2.5), seed: int = 0, bf16_llr: bool = False, log_every: int = 10,
-          callback=None, opt: Optional[torch.optim.Optimizer] = None, multi_loss: bool = False) -> List[TrainLog]:
+          callback=None, opt: Optional[torch.optim.Optimizer] = None, multi_loss: bool = False,
+          hard: Optional[torch.Tensor] = None, hard_frac: float = 0.0) -> List[TrainLog]:
     """Adam on the soft BCE (of the last iteration, or averaged over all
     iterations with `multi_loss`); every step draws `batch` codewords at
     Eb/N0 uniform in `ebn0_range`, processed in micro-batches (gradients
-    summed).  Pass `opt` to continue with an existing optimizer's state."""
+    summed).  Pass `opt` to continue with an existing optimizer's state.
+
+    `hard` [H, N] (channel LLRs of codewords a decoder failed on) with
+    `hard_frac` > 0: that share of every micro-batch is drawn from `hard`
+    (uniformly, with replacement) instead of the channel."""
     dev = model.alpha.device
     if opt is None:
         opt = torch.optim.Adam(model.parameters(), lr=lr)
@@ -165,6 +170,10 @@ def train(model: DMinSum, steps: int, batch: int = 65536, micro_batch: int = 163
             b = min(micro_batch, batch - m * micro_batch)
             eb = ebn0_range[0] + (ebn0_range[1] - ebn0_range[0]) * torch.rand(b, device=dev, generator=gen)
             llr = channel_llr(model.code, eb, gen, bf16_llr)
+            nh = int(round(hard_frac * b)) if hard is not None and hard.shape[0] else 0
+            if nh:
+                pick = torch.randint(0, hard.shape[0], (nh,), device=dev, generator=gen)
+                llr[:nh] = hard[pick]
             loss = soft_bce(model(llr, all_iters=multi_loss), model.n_tx) * (b / batch)
             loss.backward()
             total += float(loss.detach())

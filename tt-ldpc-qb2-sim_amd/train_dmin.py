@@ -45,6 +45,37 @@ def validate(model, sets):
     return out
 
 
+# Mining draws from its own seed, so that no mined codeword is one that
+# eval_dmin.py (seed 0x5EED2026) judges the decoder on.
+MINE_SEED = 0x4A2D5EED
+
+
+def mine_hard(model, args, step, hard, mined):
+    """Decode args.hard_pool fresh codewords at args.hard_ebn0 with the current
+    parameters (libldpc Profile D: plain Monte Carlo with early exit), capture
+    the ids of the failures and regenerate their channels (bf16 LLRs, the
+    channel the evaluation uses); they join the mined set, which keeps the
+    newest args.hard_keep."""
+    from ldpc_mi355x import decoder
+    p = model.export()
+    first = (step + 1) * args.hard_pool  # a new pool of codeword ids each round
+    with decoder.Engine(args.k, args.rate, max_iter=args.hard_iters, profile="D") as eng:
+        eng.set_dmin_params(p["alpha"], p["beta"], bool(p["correction"]), float(p["r_max"]))
+        eng.set_capture(args.hard_keep)
+        r = eng.run_point(args.hard_ebn0, args.hard_pool, seed=MINE_SEED, first_cw=first, chunk_cw=1 << 20)
+        ids, seen = eng.captures(args.hard_keep)
+    with decoder.Engine(args.k, args.rate, max_iter=1) as gen:
+        rows = [gen.gen_channel(MINE_SEED, int(i), 1, args.hard_ebn0) for i in ids]
+    new = None
+    if rows:
+        llr16 = torch.from_numpy(np.concatenate(rows).astype(np.int32))
+        new = (llr16 << 16).view(torch.float32).cuda()
+    if new is not None:
+        hard = new if hard is None else torch.cat([hard, new])[-args.hard_keep:]
+    return hard, {"pool": int(r.blocks), "failures": int(seen), "fer": r.fer,
+                  "set": 0 if hard is None else int(hard.shape[0])}
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--k", type=int, default=1024)
@@ -71,6 +102,14 @@ def main(argv=None):
     ap.add_argument("--init-beta", type=float, default=0.0)
     ap.add_argument("--init-from", default=None,
                     help=".npz written by this script: start from its alpha / beta (same code and parameter shape)")
+    ap.add_argument("--hard-ebn0", type=float, default=0.0,
+                    help="mine hard codewords at this Eb/N0 (0: off): every --val-every steps the current "
+                         "parameters decode --hard-pool codewords by plain Monte Carlo (libldpc Profile D, "
+                         "--hard-iters iterations, early exit) and the failures' channels join the training set")
+    ap.add_argument("--hard-pool", type=int, default=1 << 23, help="codewords decoded per mining round")
+    ap.add_argument("--hard-iters", type=int, default=100, help="iteration cap of the mining decode")
+    ap.add_argument("--hard-frac", type=float, default=0.25, help="share of each micro-batch drawn from the mined set")
+    ap.add_argument("--hard-keep", type=int, default=16384, help="most mined codewords kept (the newest)")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--out", default="dmin_params.npz")
     args = ap.parse_args(argv)
@@ -94,15 +133,19 @@ def main(argv=None):
     t0 = time.time()
     print(json.dumps({"step": 0, "val": validate(model, sets), "seconds": 0.0}), flush=True)
     done = 0
+    hard, mined = None, {}
     while done < args.steps:
         n = min(args.val_every, args.steps - done)
+        if args.hard_ebn0 > 0:
+            hard, mined = mine_hard(model, args, done, hard, mined)
         log = dmin.train(model, n, batch=args.batch, micro_batch=args.micro_batch, lr=args.lr,
                          ebn0_range=(args.ebn0_lo, args.ebn0_hi), seed=args.seed + done, bf16_llr=args.bf16_llr,
-                         log_every=max(1, n), opt=opt, multi_loss=args.multi_loss)
+                         log_every=max(1, n), opt=opt, multi_loss=args.multi_loss, hard=hard,
+                         hard_frac=args.hard_frac)
         done += n
         torch.cuda.synchronize()
         print(json.dumps({"step": done, "train_loss": log[-1].loss, "val": validate(model, sets),
-                          "seconds": time.time() - t0}), flush=True)
+                          "mined": mined, "seconds": time.time() - t0}), flush=True)
     p = model.export()
     np.savez(args.out, alpha=p["alpha"], beta=p["beta"], T=p["T"], k_info=p["k_info"], rate=p["rate"],
              correction=p["correction"], r_max=p["r_max"])

@@ -52,9 +52,9 @@ bool next_line(std::istream &f, std::string &line) {
 // .chinn: "N M [dv dc]" header, then per row "deg v1 .. vdeg" (:204-268).
 Matrix parse_matrix(const std::string &path) {
     std::ifstream f(path);
-    if (!f) throw std::runtime_error("Could not open matrix file: " + path);
+    if (!f) throw std::runtime_error("cannot read the matrix " + path);
     std::string line;
-    if (!next_line(f, line)) throw std::runtime_error("Matrix file is empty: " + path);
+    if (!next_line(f, line)) throw std::runtime_error("the matrix file " + path + " holds no lines");
     std::vector<std::string> tok;
     {
         std::stringstream ss(line);
@@ -81,7 +81,7 @@ Matrix parse_matrix(const std::string &path) {
         for (size_t i = 0; i + 1 < R.size(); ++i) {
             if (std::fabs(V[i]) <= 1e-6f) continue;
             if (R[i] == 0 || R[i] > M.m || C[i] == 0 || C[i] > M.n)
-                throw std::runtime_error("Edge index out of bounds in " + path);
+                throw std::runtime_error("an entry of " + path + " lies outside the matrix");
             rows[R[i] - 1].push_back((int32_t)(C[i] - 1));
         }
         M.row_ptr.push_back(0);
@@ -94,11 +94,11 @@ Matrix parse_matrix(const std::string &path) {
         uint32_t dv = 0, dc = 0;
         hs >> M.n >> M.m;
         if (!(hs >> dv >> dc)) {
-            if (!next_line(f, line)) throw std::runtime_error("Missing degree header in matrix file: " + path);
+            if (!next_line(f, line)) throw std::runtime_error("no degree line in the .chinn file " + path);
         }
         M.row_ptr.push_back(0);
         for (uint32_t r = 0; r < M.m; ++r) {
-            if (!next_line(f, line)) throw std::runtime_error("Unexpected EOF reading check node row " + std::to_string(r));
+            if (!next_line(f, line)) throw std::runtime_error("the .chinn file ends before check row " + std::to_string(r));
             std::stringstream rs(line);
             uint32_t deg = 0, v = 0;
             rs >> deg;
@@ -110,8 +110,8 @@ Matrix parse_matrix(const std::string &path) {
         }
     }
     if (M.n > 16384 || M.m > 8192)
-        throw std::runtime_error("Matrix dimensions exceed supported device limits: N=" + std::to_string(M.n) +
-                                 ", M=" + std::to_string(M.m));
+        throw std::runtime_error("matrix too large for the engine limits (N <= 16384, M <= 8192): N = " + std::to_string(M.n) +
+                                 ", M = " + std::to_string(M.m));
     return M;
 }
 
@@ -157,7 +157,7 @@ int main(int argc, char **argv) {
         for (uint32_t r = 0; r < H.m; ++r) maxd = std::max<uint32_t>(maxd, H.row_ptr[r + 1] - H.row_ptr[r]);
         std::cout << "[CONFIG] Matrix loaded: " << chinn << " (N=" << H.n << ", M=" << H.m
                   << ", max_check_deg=" << maxd << ")" << std::endl;
-        if (maxd > 12) throw std::runtime_error("Check node degree (" + std::to_string(maxd) + ") exceeds MAX_DEG (12)");
+        if (maxd > 12) throw std::runtime_error("check degree " + std::to_string(maxd) + " is above the engine's limit of 12");
 
         const uint64_t cores = single ? 1 : 440;
         if (max_blocks == 0 && blocks_per_core > 0) max_blocks = blocks_per_core * cores;

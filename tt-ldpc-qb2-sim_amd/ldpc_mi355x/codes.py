@@ -314,7 +314,7 @@ def read_chinn_out(path: str) -> Tuple[int, int, List[List[int]]]:
     with open(path) as f:
         lines = [ln.split() for ln in f if ln.strip() and not ln.startswith("#")]
     if not lines:
-        raise ValueError(f"Matrix file is empty: {path}")
+        raise ValueError(f"the matrix file {path} holds no lines")
     M, N = int(lines[-1][0]), int(lines[-1][1])
     rows: List[List[int]] = [[] for _ in range(M)]
     for parts in lines[:-1]:
@@ -342,7 +342,7 @@ def parse_matrix_header(path: str) -> Tuple[int, int, int, int]:
     with open(path) as f:
         lines = [ln.strip() for ln in f if ln.strip() and not ln.startswith("#")]
     if not lines:
-        raise ValueError(f"Matrix file is empty: {path}")
+        raise ValueError(f"the matrix file {path} holds no lines")
     first = lines[0].split()
     if len(first) == 3:
         M, N = int(lines[-1].split()[0]), int(lines[-1].split()[1])

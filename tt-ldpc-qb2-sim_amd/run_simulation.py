@@ -42,32 +42,32 @@ CORES_MESH = 440
 def build_parser() -> argparse.ArgumentParser:
     p = argparse.ArgumentParser(description="MI355X LDPC Monte Carlo Simulator Driver")
     p.add_argument("--chinn", "--matrix", dest="chinn", type=str, required=True,
-                   help="Path to matrix file (.chinn or .chinn.out)")
-    p.add_argument("--ebn0_start", type=float, required=True, help="Start Eb/N0 in dB")
-    p.add_argument("--ebn0_end", type=float, required=True, help="End Eb/N0 in dB")
-    p.add_argument("--ebn0_step", type=float, default=0.25, help="Eb/N0 step size in dB")
-    p.add_argument("--punctured", type=int, default=0, help="Number of punctured variable nodes")
+                   help="parity-check matrix: a .chinn.out triplet file or a legacy .chinn")
+    p.add_argument("--ebn0_start", type=float, required=True, help="first Eb/N0 point (dB)")
+    p.add_argument("--ebn0_end", type=float, required=True, help="last Eb/N0 point (dB), included")
+    p.add_argument("--ebn0_step", type=float, default=0.25, help="spacing of the Eb/N0 points (dB)")
+    p.add_argument("--punctured", type=int, default=0, help="how many trailing variables are punctured (never transmitted)")
     p.add_argument("--min_errors", "--target_errors", dest="min_errors", type=int, default=50,
-                   help="Minimum errored blocks to find per SNR point (default: 50)")
+                   help="stop a point once this many frame errors are counted (default 50)")
     p.add_argument("--max_blocks", type=int, default=None,
-                   help="Maximum total blocks across all cores (overrides max_blocks_per_core)")
+                   help="cap on codewords per point; takes precedence over --max_blocks_per_core")
     p.add_argument("--max_blocks_per_core", "--blocks_per_core", dest="max_blocks_per_core", type=int,
-                   default=100000, help="Maximum blocks to simulate per core (default: 100000)")
+                   default=100000, help="cap on codewords per point, per reference core (x 440, or x 1 with --single_core)")
     p.add_argument("--batch_size", type=int, default=1000,
-                   help="Batch size per core between error checks (default: 1000)")
+                   help="codewords per reference core between progress reports and stop checks (default 1000)")
     p.add_argument("--max_iter", "--max_iterations", dest="max_iter", type=int, default=16,
-                   help="Maximum decoder iterations per codeword (default: 16)")
+                   help="iteration cap of the decoder (default 16)")
     p.add_argument("--binary", type=str, default=None,
                    help="Engine binary speaking the JSON-lines protocol (default: in-process libldpc)")
     p.add_argument("--output", type=str, default=None,
-                   help="Output CSV file path (defaults to auto-generated from parameters)")
+                   help="CSV to write (default: a name built from the run's parameters)")
     p.add_argument("--single_core", action="store_true", help="Block accounting for one core instead of 440")
     p.add_argument("--nms", action="store_true",
-                   help="Use Normalized Min-Sum (alpha=0.75) instead of default Approximate-Min*")
-    p.add_argument("--l_max", type=float, default=0.0, help="Maximum variable node LLR magnitude (0.0 = unclipped)")
-    p.add_argument("--r_max", type=float, default=0.0, help="Maximum check node message magnitude (0.0 = unclipped)")
+                   help="decode with normalized min-sum (scale 0.75) rather than the recursive Approximate-Min*")
+    p.add_argument("--l_max", type=float, default=0.0, help="clip channel LLRs to this magnitude (0: no clip)")
+    p.add_argument("--r_max", type=float, default=0.0, help="clip check-to-variable messages to this magnitude (0: no clip)")
     p.add_argument("--l_post_max", "--v_max", dest="l_post_max", type=float, default=0.0,
-                   help="Maximum posterior variable accumulation magnitude (0.0 = unclipped)")
+                   help="clip the updated variable LLRs to this magnitude (0: no clip)")
     p.add_argument("--engine", type=str, default="scalar", choices=["scalar", "circulant"],
                    help="'scalar': the scalar_5w arithmetic on the matrix (Profile R); 'circulant': the golden "
                         "circulant arithmetic (Profile T, AR4JA r1/2 K=1024)")

@@ -120,6 +120,16 @@ def soft_bce(L: torch.Tensor, n_tx: int) -> torch.Tensor:
     return torch.nn.functional.softplus(-L[..., :n_tx]).mean()
 
 
+def soft_fer(L: torch.Tensor, n_tx: int) -> torch.Tensor:
+    """Soft frame error rate: 1 - prod over positions < n_tx of sigmoid(L),
+    i.e. 1 - exp(-sum softplus(-L)), averaged over codewords (and over
+    iterations for a [T, B, N] output).  A codeword with many unreliable bits
+    saturates at 1 and contributes little gradient, so borderline codewords
+    weigh most."""
+    s = torch.nn.functional.softplus(-L[..., :n_tx]).sum(dim=-1)
+    return (-torch.expm1(-s)).mean()
+
+
 def channel_llr(code, ebn0_db: torch.Tensor, generator: Optional[torch.Generator] = None,
                 bf16: bool = False) -> torch.Tensor:
     """BPSK LLRs of the all-zero codeword, one Eb/N0 per codeword (AWGN,
@@ -145,7 +155,7 @@ class TrainLog:
 def train(model: DMinSum, steps: int, batch: int = 65536, micro_batch: int = 16384, lr: float = 1e-3,
           ebn0_range=(1.0, 2.5), seed: int = 0, bf16_llr: bool = False, log_every: int = 10,
           callback=None, opt: Optional[torch.optim.Optimizer] = None, multi_loss: bool = False,
-          hard: Optional[torch.Tensor] = None, hard_frac: float = 0.0) -> List[TrainLog]:
+          hard: Optional[torch.Tensor] = None, hard_frac: float = 0.0, loss: str = "bce") -> List[TrainLog]:
     """Adam on the soft BCE (of the last iteration, or averaged over all
     iterations with `multi_loss`); every step draws `batch` codewords at
     Eb/N0 uniform in `ebn0_range`, processed in micro-batches (gradients
@@ -153,7 +163,8 @@ def train(model: DMinSum, steps: int, batch: int = 65536, micro_batch: int = 163
 
     `hard` [H, N] (channel LLRs of codewords a decoder failed on) with
     `hard_frac` > 0: that share of every micro-batch is drawn from `hard`
-    (uniformly, with replacement) instead of the channel."""
+    (uniformly, with replacement) instead of the channel.  `loss`: "bce"
+    (soft_bce) or "softfer" (soft_fer)."""
     dev = model.alpha.device
     if opt is None:
         opt = torch.optim.Adam(model.parameters(), lr=lr)
@@ -174,9 +185,10 @@ def train(model: DMinSum, steps: int, batch: int = 65536, micro_batch: int = 163
             if nh:
                 pick = torch.randint(0, hard.shape[0], (nh,), device=dev, generator=gen)
                 llr[:nh] = hard[pick]
-            loss = soft_bce(model(llr, all_iters=multi_loss), model.n_tx) * (b / batch)
-            loss.backward()
-            total += float(loss.detach())
+            out = model(llr, all_iters=multi_loss)
+            val = (soft_fer(out, model.n_tx) if loss == "softfer" else soft_bce(out, model.n_tx)) * (b / batch)
+            val.backward()
+            total += float(val.detach())
         opt.step()
         if step % log_every == 0 or step == steps - 1:
             log.append(TrainLog(step, total, time.time() - t0))

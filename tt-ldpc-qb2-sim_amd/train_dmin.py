@@ -92,7 +92,10 @@ def main(argv=None):
     ap.add_argument("--correction", action="store_true", help="Approx-Min* correction on the min-sum magnitude")
     ap.add_argument("--per-iteration", action="store_true", help="one parameter row per iteration")
     ap.add_argument("--bf16-llr", action="store_true", help="round channel LLRs to bf16")
-    ap.add_argument("--multi-loss", action="store_true", help="average the BCE over every iteration's output")
+    ap.add_argument("--multi-loss", action="store_true", help="average the loss over every iteration's output")
+    ap.add_argument("--loss", choices=["bce", "softfer"], default="bce",
+                    help="bce: mean binary cross-entropy of the soft output; softfer: soft frame error rate, "
+                         "1 - prod sigmoid(L) per codeword")
     ap.add_argument("--trunc", type=int, default=0,
                     help="truncate backpropagation every TRUNC iterations (0: through all T)")
     ap.add_argument("--val-ebn0", type=float, nargs="*", default=[1.5, 2.0])
@@ -141,7 +144,7 @@ def main(argv=None):
         log = dmin.train(model, n, batch=args.batch, micro_batch=args.micro_batch, lr=args.lr,
                          ebn0_range=(args.ebn0_lo, args.ebn0_hi), seed=args.seed + done, bf16_llr=args.bf16_llr,
                          log_every=max(1, n), opt=opt, multi_loss=args.multi_loss, hard=hard,
-                         hard_frac=args.hard_frac)
+                         hard_frac=args.hard_frac, loss=args.loss)
         done += n
         torch.cuda.synchronize()
         print(json.dumps({"step": done, "train_loss": log[-1].loss, "val": validate(model, sets),

@@ -98,3 +98,23 @@ def test_schedule_table_matches_compiled_schedule(code12):
     for i, (first, slots) in enumerate(sched):
         assert tab[i, 0] == len(slots) and tab[i, 1] == 128
         assert [(tab[i, 2 + 2 * d], tab[i, 3 + 2 * d]) for d in range(len(slots))] == slots
+
+
+def test_soft_fer_is_one_minus_the_product_of_bit_probabilities():
+    """soft_fer (1 - exp(-sum softplus(-L))) against the product of the
+    per-bit probabilities sigmoid(L), formed in float64; positions past n_tx
+    do not count; hopeless and clean codewords sit at 1 and 0."""
+    from ldpc_mi355x import dmin as D
+    rng = np.random.default_rng(4)
+    n_tx, N = 40, 48
+    L = torch.from_numpy(rng.normal(3.0, 4.0, (6, N))).float()
+    L[:, n_tx:] = -50.0  # punctured positions: ignored
+    direct = (1.0 - torch.sigmoid(L[:, :n_tx].double()).prod(dim=1)).mean()
+    assert abs(float(D.soft_fer(L, n_tx)) - float(direct)) < 1e-6
+    clean = torch.full((3, N), 30.0)
+    hopeless = torch.full((3, N), -30.0)
+    assert float(D.soft_fer(clean, n_tx)) < 1e-9
+    assert abs(float(D.soft_fer(hopeless, n_tx)) - 1.0) < 1e-9
+    # [T, B, N]: the mean over iterations as well
+    stacked = torch.stack([clean, hopeless])
+    assert abs(float(D.soft_fer(stacked, n_tx)) - 0.5) < 1e-9

@@ -1,0 +1,68 @@
+"""Monte Carlo far outside the waterfall, against the oracle (needs an MI355X).
+
+Far below it every codeword runs to the iteration cap, so the failure path of
+each kernel (bit-error count of every codeword, histogram bin 0, the cap in
+the iteration sum) carries the whole tally; far above it nearly every
+codeword converges at the first or second iteration, so the latch and the
+refill run once per codeword.  Counters and histograms bit for bit, for the
+specialised r1/2 K=1024 kernel, a wave-mode and a group-mode nine-code kernel,
+and Profile R.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import channel_params
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+MATRIX = os.path.join(HERE, "golden", "AR4JA_r45_4c_128c_r12.chinn.out")
+RATE = {"1/2": 0.5, "2/3": 2.0 / 3.0, "4/5": 0.8}
+
+
+def _case(msg):
+    print(f"[gpu-case] {msg}", flush=True)
+
+
+def _same(r, o):
+    assert (r.blocks, r.frame_errors, r.bit_errors, r.iter_sum) == (o["blocks"], o["frame_errors"],
+                                                                     o["bit_errors"], o["iter_sum"])
+    np.testing.assert_array_equal(r.hist, o["hist"])
+
+
+@pytest.mark.parametrize("k,rate", [(1024, "1/2"), (1024, "4/5"), (4096, "1/2")])
+@pytest.mark.parametrize("ebn0,max_iter,n", [(-3.0, 12, 768), (12.0, 200, 2048)])
+def test_profile_t_far_from_the_waterfall(oracle, k, rate, ebn0, max_iter, n):
+    from ldpc_mi355x import codes, decoder
+    n = n if k == 1024 else n // 4
+    _case(f"Profile T K={k} r{rate} {ebn0} dB max_iter {max_iter} n {n}")
+    with decoder.Engine(k, rate, max_iter=max_iter, beta=0.9375) as eng:
+        r = eng.run_point(ebn0, n, seed=0xE1, first_cw=3, chunk_cw=n)
+    mu, sg = channel_params(ebn0, RATE[rate])
+    o = oracle.run_point_t(codes.ar4ja(k, rate), 0xE1, 3, n, mu, sg, max_iter, 0.9375)
+    _same(r, o)
+    if ebn0 < 0:
+        assert r.frame_errors >= 0.99 * n and r.hist[0] == r.frame_errors
+    else:
+        assert r.frame_errors == 0 and r.hist[1] + r.hist[2] >= 0.99 * n
+
+
+@pytest.mark.parametrize("profile", ["R", "R_NMS"])
+@pytest.mark.parametrize("ebn0,max_iter,n", [(-3.0, 12, 768), (12.0, 60, 2048)])
+def test_profile_r_far_from_the_waterfall(oracle, profile, ebn0, max_iter, n):
+    from ldpc_mi355x import codes, decoder
+    M, N, rows = codes.read_chinn_out(MATRIX)
+    _case(f"Profile {profile} {ebn0} dB max_iter {max_iter} n {n}")
+    with decoder.Engine(profile=profile, matrix=(rows, N), n_punct=512, max_iter=max_iter) as eng:
+        r = eng.run_point(ebn0, n, seed=0xE2, first_cw=3, chunk_cw=n)
+        mu, sg = eng.channel_params(ebn0)
+    o = oracle.run_point_r(N, rows, 2048, 0xE2, 3, n, mu, sg, np.float32(0.0), max_iter, profile == "R_NMS", 0.0,
+                           0.0)
+    _same(r, o)
+    if ebn0 < 0:
+        assert r.frame_errors >= 0.99 * n
+    else:
+        # the syndrome is first checked after the third iteration
+        assert r.frame_errors == 0 and r.hist[3] >= 0.99 * n

@@ -92,3 +92,23 @@ def test_profile_d_far_from_the_waterfall(oracle, k, ebn0, max_iter, n):
         assert r.frame_errors >= 0.99 * n
     else:
         assert r.frame_errors == 0
+
+
+@pytest.mark.parametrize("ebn0,max_iter,n", [(-3.0, 12, 768), (12.0, 200, 2048)])
+def test_random_codewords_far_from_the_waterfall(oracle, code12, ebn0, max_iter, n):
+    """Random information bits (r1/2 K=1024): failures counted against the sent
+    bits at -3 dB, clean convergence to the sent codeword at 12 dB."""
+    from ldpc_mi355x import decoder
+    _case(f"random codewords {ebn0} dB max_iter {max_iter} n {n}")
+    with decoder.Engine(1024, "1/2", max_iter=max_iter, beta=0.9375) as eng:
+        eng.set_codeword_mode(True)
+        r = eng.run_point(ebn0, n, seed=0xE4, first_cw=3, chunk_cw=n)
+    mu, sg = channel_params(ebn0)
+    o = oracle.run_point_t_cw(code12, 0xE4, 3, n, mu, sg, max_iter, 0.9375)
+    _same(r, o)
+    if ebn0 < 0:
+        # a failed frame keeps about the raw channel's error rate on its information bits: Q(mu / sigma) =
+        # Q(sqrt(2 R g)) = Q(0.707) = 0.24 at -3 dB
+        assert r.frame_errors >= 0.99 * n and 0.18 < r.bit_errors / (1024 * r.frame_errors) < 0.30
+    else:
+        assert r.frame_errors == 0 and r.bit_errors == 0

@@ -112,3 +112,22 @@ def test_random_codewords_far_from_the_waterfall(oracle, code12, ebn0, max_iter,
         assert r.frame_errors >= 0.99 * n and 0.18 < r.bit_errors / (1024 * r.frame_errors) < 0.30
     else:
         assert r.frame_errors == 0 and r.bit_errors == 0
+
+
+def test_batch_with_codes_far_from_the_waterfall():
+    """One graph mixing codes far below, far above and inside their waterfalls
+    equals the per-code runs counter for counter."""
+    from ldpc_mi355x import decoder
+    codes3 = [(1024, "4/5"), (4096, "1/2"), (16384, "2/3")]
+    ebn0 = [-3.0, 12.0, 1.6]
+    n_cw = [512, 96, 24]
+    _case(f"batch {codes3} at {ebn0}")
+    with decoder.Batch(codes3, max_iter=16) as b:
+        got = b.run(ebn0, n_cw, seed=[7, 8, 9], first_cw=2, chunk_cw=[n // 2 for n in n_cw])
+    for i, (k, rate) in enumerate(codes3):
+        with decoder.Engine(k, rate, max_iter=16) as eng:
+            ref = eng.run_point(ebn0[i], n_cw[i], seed=7 + i, first_cw=2, chunk_cw=n_cw[i])
+        assert (got[i].blocks, got[i].frame_errors, got[i].bit_errors, got[i].iter_sum) == \
+            (ref.blocks, ref.frame_errors, ref.bit_errors, ref.iter_sum), (k, rate)
+        np.testing.assert_array_equal(got[i].hist, ref.hist)
+    assert got[0].frame_errors == n_cw[0] and got[1].frame_errors == 0

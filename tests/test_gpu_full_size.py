@@ -4,7 +4,8 @@ Monte Carlo kernel, counters and the whole iteration histogram equal to the
 oracle's; a bit-exact decode of a 1e5-codeword LLR dump at 2.5 dB (final
 LLRs, iterations, convergence); and the same 1e5-codeword Monte Carlo check
 for Profile R (the scalar_5w arithmetic run_simulation.py drives) on the
-reference's .chinn.out."""
+reference's .chinn.out; and the large codes of the nine-code batch at their
+batch Eb/N0 and 200 iterations."""
 import os
 
 import numpy as np
@@ -56,3 +57,22 @@ def test_profile_r_monte_carlo_1e5_codewords_matches_oracle(oracle, ebn0):
         (exp["blocks"], exp["frame_errors"], exp["bit_errors"], exp["iter_sum"])
     np.testing.assert_array_equal(got.hist, exp["hist"])
     print(f"  FER {got.fer:.3e}  BER {got.ber:.3e}  mean iterations {got.mean_iterations:.3f}", flush=True)
+
+
+RATE = {"1/2": 0.5, "2/3": 2.0 / 3.0, "4/5": 0.8}
+
+
+@pytest.mark.parametrize("k,rate,ebn0,n", [(16384, "1/2", 1.2, 2048), (16384, "2/3", 2.0, 2048),
+                                           (16384, "4/5", 3.0, 2048), (4096, "4/5", 3.0, 8192)])
+def test_large_codes_at_their_batch_points_match_oracle(oracle, k, rate, ebn0, n):
+    """The nine-code batch's critical-path codes at their own Eb/N0 and 200
+    iterations (library-default kernels): counters and histogram bit for bit."""
+    mu, sg = channel_params(ebn0, RATE[rate])
+    print(f"K={k} r{rate} {n} codewords at {ebn0} dB, 200 iterations", flush=True)
+    with decoder.Engine(k, rate, max_iter=200) as eng:
+        got = eng.run_point(ebn0, n, seed=0x5EED2026, first_cw=11, chunk_cw=n)
+    exp = oracle.run_point_t(codes.ar4ja(k, rate), 0x5EED2026, 11, n, mu, sg, 200, 0.9375)
+    assert (got.blocks, got.frame_errors, got.bit_errors, got.iter_sum) == \
+        (exp["blocks"], exp["frame_errors"], exp["bit_errors"], exp["iter_sum"])
+    np.testing.assert_array_equal(got.hist, exp["hist"])
+    print(f"  mean iterations {got.mean_iterations:.3f}", flush=True)

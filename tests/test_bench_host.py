@@ -17,6 +17,18 @@ def test_work_model_matches_the_degree_table():
     assert bench.ops_per_cwi(16384, "4/5") == 16 * 128 * (86 + 2 * bench.ops_per_check(18))
 
 
+def test_profile_r_work_model_on_the_chinn_out():
+    """23 lane-ops per edge plus 13 per fold step, d - 1 fold steps per row of
+    degree d: 203 for degree 6, 95 for degree 3, 256,512 for the .chinn.out's
+    1024 degree-6 and 512 degree-3 rows."""
+    from ldpc_mi355x import codes
+    assert bench.ops_per_row_r(6) == 203 and bench.ops_per_row_r(3) == 95
+    _, _, rows = codes.read_chinn_out(bench.SCALAR_MATRIX)
+    degs = sorted(len(r) for r in rows)
+    assert degs.count(3) == 512 and degs.count(6) == 1024 and len(degs) == 1536
+    assert sum(bench.ops_per_row_r(len(r)) for r in rows) == 256_512
+
+
 def test_fer_match_is_an_exact_two_sample_poisson_test():
     scipy = pytest.importorskip("scipy")  # noqa: F841
     same = bench.fer_match(57, 20_971_520, {"blocks": 368_640, "frame_errors": 1})

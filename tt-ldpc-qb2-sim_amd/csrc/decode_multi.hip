@@ -108,13 +108,17 @@ struct Cfg {
     //  * rate 4/5, C = 1, group mode with at most 256 threads (K = 4096):
     //    80 message dwords and the degree-18 working set take 177 VGPRs,
     //    2 waves per SIMD; the degree-3 layers and the first two degree-18
-    //    layers go to LDS (26 dwords per thread): 161 VGPRs, 3 waves per SIMD,
-    //    6% faster at 3.0 dB.  (K = 1024 r4/5 in wave mode reaches 3 waves only
-    //    with a forced 168-VGPR budget, which cost more than it gained at
-    //    3.5 dB, so it keeps every message in VGPRs.)
-    // Waves per SIMD the compiler must fit.  Rate 4/5 in wave mode with one
-    // check per thread (K = 1024 r4/5) compiles to 171 VGPRs, 2 waves per
-    // SIMD; held to 3 waves it fits in 167 with no spill.
+    //    layers go to LDS (26 dwords per thread): about 148 VGPRs, 3 waves per
+    //    SIMD, 6% faster at 3.0 dB;
+    //  * rate 4/5, C = 1, group mode with 512 threads (K = 16384): the
+    //    degree-3 layers and the first degree-18 layer go to LDS (17 dwords),
+    //    the degree-18 layers recompute their store addresses after the tree,
+    //    and the kernel is held to 4 waves per SIMD (128 VGPRs, a few dwords
+    //    spilled outside the iteration loop): two workgroups per CU.
+    // Waves per SIMD the compiler must fit: the K = 16384 r4/5 layout above
+    // at 4; rate 4/5 in wave mode with one check per thread (K = 1024 r4/5),
+    // which compiles to 171 VGPRs (2 waves per SIMD) unconstrained, at 3,
+    // where it fits in 167 with no spill and keeps every message in VGPRs.
     static constexpr bool LDS_R45B = LDPC_RID == 2 && C == 1 && !WAVE && THREADS == 512;
     static constexpr bool RECOMPUTE = LDS_R45B;
     static constexpr int MIN_WAVES = (LDPC_RID == 2 && WAVE && C == 1) ? 3 : LDS_R45B ? 4 : 1;

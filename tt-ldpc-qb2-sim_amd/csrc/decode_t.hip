@@ -366,8 +366,9 @@ __device__ __forceinline__ void wave_cw_errors(const uint16_t *Ls, const uint32_
 // Work distribution: every codeword of a launch comes from a device queue,
 // one atomic per codeword.  Waves that drew codewords needing few iterations
 // (or that run on a faster CU) take more of them.  A round-robin static
-// share of 7/8 of the launch ran 13% slower on the r1/2 K=1024 point: the
-// per-wave work it fixes in advance cannot follow the wave's real speed.
+// share of 7/8 of the launch took 17% more time on the r1/2 K=1024 point
+// (60.3 against 70.5 Gsps): the per-wave work it fixes in advance cannot
+// follow the wave's real speed.
 __device__ __forceinline__ uint32_t next_codeword(uint32_t *queue, int lane) {
     uint32_t v = 0;
     if (lane == 0) v = atomicAdd(queue, 1u);

@@ -314,8 +314,11 @@ int ldpc_get_captures(ldpc_ctx *ctx, uint64_t *cw_ids, uint32_t cap, uint64_t *n
 
 /* Profile D parameters: alpha and beta are [groups][n_params] fp32 (row t
  * for iteration t, the last row beyond; groups = 1 shares one row), n_params
- * = the sum of the layer degrees (n_edges / z); correction != 0 applies the
- * Approx-Min* correction to the min-sum magnitude; r_max > 0 caps message
+ * = the sum of the layer degrees (n_edges / z); correction 0 is plain
+ * min-sum, 1 applies the Approx-Min* correction to the minimum the other
+ * edges get (from the two smallest magnitudes), 2 keeps three minima and
+ * gives every edge min* of the two smallest magnitudes among the others (not
+ * for K=16384 r1/2: LDPC_ERR_UNSUPPORTED); r_max > 0 caps message
  * magnitudes.  Copied to the device; required before Profile D work.  No
  * reference counterpart (the trained decoder's parameters). */
 int ldpc_set_dmin_params(ldpc_ctx *ctx, const float *alpha, const float *beta, uint32_t groups, uint32_t n_params,

@@ -17,8 +17,11 @@ Decoder (per layer of the AR4JA layered schedule, checks in parallel):
     g_d   = min2                          if d == idx1
           = min1                          otherwise (min-sum), or
           = max(min1 - corr(min2 - min1), 0)   with the Approx-Min* correction
-            corr(x) = max(C1 + K1 x, C2 + K2 x, 0), bf16-truncated constants
-            of the reference (sfpi_host_emu.h:566-584)
+            (corr = 1), corr(x) = max(C1 + K1 x, C2 + K2 x, 0), bf16-truncated
+            constants of the reference (sfpi_host_emu.h:566-584)
+    with three minima (corr = 2; idx3 = first argmin without idx1, idx2):
+      g_d = amin(min2, min3) if d == idx1, amin(min1, min3) if d == idx2,
+            amin(min1, min2) otherwise,  amin(lo, hi) = max(lo - corr(hi - lo), 0)
     z_d   = alpha_p(d) * g_d - beta_p(d)          (learnable, per layer slot)
     r_d   = copysign(max(z_d, 0), s ^ sign(q_d)),  s = XOR of all sign bits
     (with r_max > 0 the magnitude is min(max(z_d, 0), r_max))
@@ -55,6 +58,15 @@ def n_params(code) -> int:
     return sum(len(s) for _, s in code.layers)
 
 
+def _amin(lo, hi, dt):
+    """Approx-Min* of two magnitudes lo <= hi: max(lo - corr(hi - lo), 0)."""
+    x = (hi - lo).astype(dt)
+    l1 = (dt(K1) * x).astype(dt) + dt(C1)
+    l2 = (dt(K2) * x).astype(dt) + dt(C2)
+    c = np.maximum(np.maximum(l1, l2), dt(0))
+    return np.maximum((lo - c).astype(dt), dt(0))
+
+
 def _layer(L, R, lay, alpha_row, beta_row, corr, dt, tape=None, r_max=0.0):
     deg, z, var, e_off, p_off = lay
     B = L.shape[0]
@@ -69,16 +81,18 @@ def _layer(L, R, lay, alpha_row, beta_row, corr, dt, tape=None, r_max=0.0):
     np.put_along_axis(a2, idx1[:, None, :], np.inf, 1)
     idx2 = np.argmin(a2, axis=1)
     min2 = np.take_along_axis(a, idx2[:, None, :], 1)[:, 0, :]
-    if corr:
-        x = (min2 - min1).astype(dt)
-        l1 = (dt(K1) * x).astype(dt) + dt(C1)
-        l2 = (dt(K2) * x).astype(dt) + dt(C2)
-        c = np.maximum(np.maximum(l1, l2), dt(0))
-        other = np.maximum((min1 - c).astype(dt), dt(0))
-    else:
-        other = min1
     d_ix = np.arange(deg)[None, :, None]
-    g = np.where(d_ix == idx1[:, None, :], min2[:, None, :], other[:, None, :]).astype(dt)
+    if corr == 2:
+        a3 = a2.copy()
+        np.put_along_axis(a3, idx2[:, None, :], np.inf, 1)
+        idx3 = np.argmin(a3, axis=1)
+        min3 = np.take_along_axis(a, idx3[:, None, :], 1)[:, 0, :]
+        o12, o13, o23 = _amin(min1, min2, dt), _amin(min1, min3, dt), _amin(min2, min3, dt)
+        g = np.where(d_ix == idx1[:, None, :], o23[:, None, :],
+                     np.where(d_ix == idx2[:, None, :], o13[:, None, :], o12[:, None, :])).astype(dt)
+    else:
+        other = _amin(min1, min2, dt) if corr else min1
+        g = np.where(d_ix == idx1[:, None, :], min2[:, None, :], other[:, None, :]).astype(dt)
     al = alpha_row[p_off:p_off + deg].astype(dt)[None, :, None]
     be = beta_row[p_off:p_off + deg].astype(dt)[None, :, None]
     zz = ((al * g).astype(dt) - be).astype(dt)

@@ -3,7 +3,7 @@
 
 Same arithmetic as oracle/dmin_oracle.py, written with differentiable torch
 ops whose subgradients follow the kernel's rules: first-index argmin / second
-minimum, where() selecting one branch (relu at z > 0, correction line 1 when
+(/ third) minimum, where() selecting one branch (relu at z > 0, correction line 1 when
 l1 >= l2, clamp at a positive corrected minimum), d|q|/dq = sgn(q) with
 sgn(0) = 0, sign bits carrying no gradient.
 """
@@ -24,7 +24,20 @@ def layers_of(code, device):
     return out, e_off
 
 
+def _amin(lo, hi):
+    """Approx-Min* of two magnitudes lo <= hi: max(lo - corr(hi - lo), 0),
+    the gradient through the selected line and the positive branch only."""
+    x = hi - lo
+    l1 = K1 * x + C1
+    l2 = K2 * x + C2
+    lm = torch.where(l1 >= l2, l1, l2)
+    c = torch.where(lm > 0, lm, torch.zeros_like(lm))
+    o = lo - c
+    return torch.where(o > 0, o, torch.zeros_like(o))
+
+
 def forward(code, llr, alpha, beta, T, corr=False, r_max=0.0, all_iters=False, trunc=0):
+    """corr: 0 / False min-sum, 1 / True two-minimum correction, 2 three minima."""
     lays, E = layers_of(code, llr.device)
     B = llr.shape[0]
     L = llr
@@ -41,18 +54,14 @@ def forward(code, llr, alpha, beta, T, corr=False, r_max=0.0, all_iters=False, t
             a2 = a.detach().scatter(1, idx1, float("inf"))
             idx2 = a2.argmin(dim=1, keepdim=True)
             m2 = a.gather(1, idx2)
-            if corr:
-                x = m2 - m1
-                l1 = K1 * x + C1
-                l2 = K2 * x + C2
-                lm = torch.where(l1 >= l2, l1, l2)
-                c = torch.where(lm > 0, lm, torch.zeros_like(lm))
-                o = m1 - c
-                other = torch.where(o > 0, o, torch.zeros_like(o))
-            else:
-                other = m1
             dix = torch.arange(deg, device=llr.device)[None, :, None]
-            gm = torch.where(dix == idx1, m2, other)
+            if corr == 2:
+                a3 = a2.scatter(1, idx2, float("inf"))
+                idx3 = a3.argmin(dim=1, keepdim=True)
+                m3 = a.gather(1, idx3)
+                gm = torch.where(dix == idx1, _amin(m2, m3), torch.where(dix == idx2, _amin(m1, m3), _amin(m1, m2)))
+            else:
+                gm = torch.where(dix == idx1, m2, _amin(m1, m2) if corr else m1)
             zz = alpha[g, p_off:p_off + deg][None, :, None] * gm - beta[g, p_off:p_off + deg][None, :, None]
             mag = torch.where(zz > 0, zz, torch.zeros_like(zz))
             if r_max > 0:

@@ -31,7 +31,7 @@ def rand_params(code, G, seed):
     return (rng.uniform(0.7, 1.0, (G, P)).astype(np.float32), rng.uniform(0.05, 0.3, (G, P)).astype(np.float32))
 
 
-@pytest.mark.parametrize("corr,r_max", [(False, 0.0), (True, 0.0), (True, 2.0)])
+@pytest.mark.parametrize("corr,r_max", [(False, 0.0), (True, 0.0), (True, 2.0), (2, 0.0), (2, 2.0)])
 def test_oracle_and_torch_reference_forward_agree(code12, corr, r_max):
     llr = llr_batch(code12, 8, 1.5, 1)
     a, b = rand_params(code12, 3, 2)
@@ -69,7 +69,7 @@ def test_unit_parameters_are_plain_layered_min_sum(code12):
     assert L.shape == L_one.shape
 
 
-@pytest.mark.parametrize("corr,r_max", [(False, 0.0), (True, 0.0), (False, 2.0)])
+@pytest.mark.parametrize("corr,r_max", [(False, 0.0), (True, 0.0), (False, 2.0), (2, 0.0)])
 def test_torch_reference_gradient_matches_finite_differences(code12, corr, r_max):
     llr = llr_batch(code12, 4, 1.5, 5)
     a, b = rand_params(code12, 1, 6)
@@ -89,6 +89,68 @@ def test_torch_reference_gradient_matches_finite_differences(code12, corr, r_max
             an = grad[g, p].item()
             assert any(abs(f[k] - an) <= 1e-4 * abs(an) + 1e-10 for f in fds), (which, p, an, [f[k] for f in fds])
     assert at.grad.abs().max() > 1e-4 and bt.grad.abs().max() > 1e-4
+
+
+def _amin_scalar(lo, hi):
+    """min*(lo, hi) for lo <= hi with the correction lines, float32 per op."""
+    f = np.float32
+    x = f(hi) - f(lo)
+    c = max(f(f(DO.K1) * x) + f(DO.C1), f(f(DO.K2) * x) + f(DO.C2), f(0))
+    return max(f(f(lo) - c), f(0))
+
+
+@pytest.mark.parametrize("k,rate", [(1024, "1/2"), (1024, "4/5")])
+def test_three_minima_give_each_edge_the_two_smallest_others(k, rate):
+    """Three-minimum correction (corr = 2), one layer at alpha = 1, beta = 0,
+    against a direct per-edge computation: the edge's magnitude is min* of
+    the two smallest magnitudes among the OTHER edges of its check (so never
+    its own), with the product sign of the others.  The magnitudes take few
+    values, so ties are common: the result must not depend on which of two
+    tied edges counts as the first, second or third minimum."""
+    code = codes.ar4ja(k, rate)
+    sch = DO.schedule(code)
+    deg, z, var, _, _ = sch[1]
+    rng = np.random.default_rng(7)
+    B = 3
+    llr = np.zeros((B, code.n), np.float32)
+    llr[:, :] = rng.choice(np.array([-2.5, -1.0, -0.5, 0.5, 1.0, 1.25, 2.5, 4.0], np.float32), (B, code.n))
+    P = DO.n_params(code)
+    L = llr.copy()
+    R = np.zeros((B, sum(l[0] * l[1] for l in sch)), np.float32)
+    DO._layer(L, R, sch[1], np.ones(P, np.float32), np.zeros(P, np.float32), 2, np.float32)
+    q = llr[:, var]                              # R = 0 before the first pass: q = L
+    for b in range(B):
+        for j in range(z):
+            for d in range(deg):
+                others = [q[b, e, j] for e in range(deg) if e != d]
+                m = sorted(abs(float(v)) for v in others)
+                mag = _amin_scalar(m[0], m[1])
+                neg = sum(v < 0 or (v == 0 and np.signbit(v)) for v in others) % 2
+                exp = np.float32(q[b, d, j] + (-mag if neg else mag))
+                assert L[b, var[d, j]] == exp, (b, j, d)
+
+
+def test_three_minima_on_a_degree_3_check_is_the_golden_pairwise_min_star():
+    """On a degree-3 layer each edge's two smallest others are simply the two
+    other edges, so corr = 2 at alpha = 1, beta = 0 equals the golden tree's
+    degree-3 update without damping: r_d = min*(|q_e|, |q_f|), e, f != d."""
+    code = codes.ar4ja(1024, "1/2")
+    sch = DO.schedule(code)
+    lay = next(l for l in sch if l[0] == 3)
+    deg, z, var, _, _ = lay
+    rng = np.random.default_rng(9)
+    llr = (rng.standard_normal((4, code.n)) * 3).astype(np.float32)
+    P = DO.n_params(code)
+    L = llr.copy()
+    R = np.zeros((4, sum(l[0] * l[1] for l in sch)), np.float32)
+    DO._layer(L, R, lay, np.ones(P, np.float32), np.zeros(P, np.float32), 2, np.float32)
+    q = llr[:, var]
+    a = np.abs(q)
+    for d, (e, f) in enumerate([(1, 2), (0, 2), (0, 1)]):
+        lo, hi = np.minimum(a[:, e], a[:, f]), np.maximum(a[:, e], a[:, f])
+        mag = np.vectorize(_amin_scalar, otypes=[np.float32])(lo, hi)
+        sign = np.where(np.signbit(q[:, e]) ^ np.signbit(q[:, f]), np.float32(-1), np.float32(1))
+        np.testing.assert_array_equal(L[:, var[d]], (q[:, d] + sign * mag).astype(np.float32))
 
 
 def test_schedule_table_matches_compiled_schedule(code12):

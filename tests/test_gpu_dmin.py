@@ -49,7 +49,7 @@ def run_ext(code, llr, a, b, T, corr, ckpt_every=4, grad=True, r_max=0.0):
 
 @pytest.mark.parametrize("k,rate,B,T", [(1024, "1/2", 64, 4), (1024, "4/5", 32, 3), (4096, "2/3", 8, 2),
                                         (16384, "1/2", 4, 2)])
-@pytest.mark.parametrize("corr,r_max", [(False, 0.0), (True, 0.0), (False, 3.0)])
+@pytest.mark.parametrize("corr,r_max", [(False, 0.0), (True, 0.0), (False, 3.0), (2, 0.0)])
 def test_forward_matches_oracle_bit_for_bit(k, rate, B, T, corr, r_max):
     code = codes.ar4ja(k, rate)
     llr = llr_batch(code, B, 1.0 + 0.8 * (rate != "1/2"), 3)
@@ -60,7 +60,7 @@ def test_forward_matches_oracle_bit_for_bit(k, rate, B, T, corr, r_max):
     np.testing.assert_array_equal(got.view(np.uint32), exp.view(np.uint32))
 
 
-@pytest.mark.parametrize("corr,r_max", [(False, 0.0), (True, 0.0), (True, 2.5)])
+@pytest.mark.parametrize("corr,r_max", [(False, 0.0), (True, 0.0), (True, 2.5), (2, 0.0), (2, 2.5)])
 @pytest.mark.parametrize("G", [1, 5])
 def test_backward_matches_torch_autograd(corr, r_max, G):
     code = codes.ar4ja(1024, "1/2")
@@ -152,6 +152,27 @@ def test_checkpoint_interval_changes_nothing():
             np.testing.assert_array_equal(x, y)
 
 
+@pytest.mark.parametrize("corr", [0, 1, 2])
+def test_no_grad_forward_equals_the_autograd_forward(corr):
+    """Under torch.no_grad the module skips the autograd.Function (and its
+    checkpoints); the soft output is the same, bit for bit, for the last
+    iteration and for every iteration."""
+    D = dmin()
+    model = D.DMinSum(1024, "1/2", T=5, ckpt_every=4, correction=corr)
+    code = model.code
+    a, b = rand_params(code, 1, 51)
+    with torch.no_grad():
+        model.alpha.copy_(torch.from_numpy(a))
+        model.beta.copy_(torch.from_numpy(b))
+    llr = torch.from_numpy(llr_batch(code, 16, 1.4, 52)).cuda()
+    for all_iters in (False, True):
+        with torch.no_grad():
+            quick = model(llr, all_iters=all_iters)
+        full = model(llr, all_iters=all_iters)
+        assert full.requires_grad and not quick.requires_grad
+        np.testing.assert_array_equal(quick.cpu().numpy(), full.detach().cpu().numpy())
+
+
 def test_training_draws_mined_codewords():
     """hard_frac = 1 with a one-codeword hard set: every row of the batch is
     that codeword, so the first step's loss (taken before the update) is its
@@ -199,19 +220,21 @@ def test_training_steps_run():
     (16384, "1/2", 2, 3, 5, 1.0),      # z = 2048, the large training configuration's code
     (16384, "1/2", 2, 6, 60, 1.0),     # checkpoints every 5 iterations (60 layer steps), as trained at T = 50
 ])
-def test_backward_other_codes_matches_torch_autograd(k, rate, B, T, ckpt_every, ebn0):
+@pytest.mark.parametrize("corr", [1, 2])
+def test_backward_other_codes_matches_torch_autograd(k, rate, B, T, ckpt_every, ebn0, corr):
     """The backward for degree 10 and 18 and for K=16384 (z = 2048), with the
-    min* correction and checkpoint intervals that are not multiples of the 12
-    layers per iteration, so that every recompute starts mid-iteration."""
+    min* correction (two and three minima) and checkpoint intervals that are
+    not multiples of the 12 layers per iteration, so that every recompute
+    starts mid-iteration."""
     code = codes.ar4ja(k, rate)
     llr = llr_batch(code, B, ebn0, 41)
     a, b = rand_params(code, 1, 42)
-    print(f"backward K={k} rate={rate} B={B} T={T} ckpt_every={ckpt_every}", flush=True)
-    L, gx, ga, gb = run_ext(code, llr, a, b, T, True, ckpt_every=ckpt_every)
+    print(f"backward K={k} rate={rate} B={B} T={T} ckpt_every={ckpt_every} corr={corr}", flush=True)
+    L, gx, ga, gb = run_ext(code, llr, a, b, T, corr, ckpt_every=ckpt_every)
     x = torch.tensor(llr, device="cuda", requires_grad=True)
     at = torch.tensor(a, device="cuda", requires_grad=True)
     bt = torch.tensor(b, device="cuda", requires_grad=True)
-    Lr = REF.forward(code, x, at, bt, T, True, 0.0)
+    Lr = REF.forward(code, x, at, bt, T, corr, 0.0)
     REF.loss(Lr, code.n_tx).backward()
     np.testing.assert_array_equal(L.view(np.uint32), Lr.detach().cpu().numpy().view(np.uint32))
     np.testing.assert_allclose(gx, x.grad.cpu().numpy(), rtol=1e-4, atol=1e-9)
@@ -219,7 +242,7 @@ def test_backward_other_codes_matches_torch_autograd(k, rate, B, T, ckpt_every, 
     np.testing.assert_allclose(gb, bt.grad.cpu().numpy(), rtol=1e-4, atol=1e-8)
     assert np.abs(ga).max() > 0 and np.abs(gb).max() > 0
     # the checkpoint interval changes nothing
-    L1, gx1, ga1, gb1 = run_ext(code, llr, a, b, T, True, ckpt_every=1)
+    L1, gx1, ga1, gb1 = run_ext(code, llr, a, b, T, corr, ckpt_every=1)
     for p, q in ((L, L1), (gx, gx1), (ga, ga1), (gb, gb1)):
         np.testing.assert_array_equal(p, q)
 

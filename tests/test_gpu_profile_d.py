@@ -24,9 +24,14 @@ def params(code, G, seed):
     return rng.uniform(0.7, 0.95, (G, P)).astype(np.float32), rng.uniform(0.0, 0.4, (G, P)).astype(np.float32)
 
 
-@pytest.mark.parametrize("k,rate,ebn0", CASES)
-@pytest.mark.parametrize("G,corr,r_max", [(1, False, 0.0), (8, True, 0.0), (1, False, 6.0), (8, True, 6.0)])
-def test_decode_matches_oracle(oracle, k, rate, ebn0, G, corr, r_max):
+# the three-minimum correction (corr = 2) runs on every code but K=16384 r1/2
+DECODE_CASES = [(c, m) for c in CASES for m in [(1, False, 0.0), (8, True, 0.0), (1, False, 6.0), (8, True, 6.0)]] + \
+    [(c, m) for c in CASES if c[0] != 16384 for m in [(8, 2, 0.0), (1, 2, 6.0)]]
+
+
+@pytest.mark.parametrize("case,mode", DECODE_CASES)
+def test_decode_matches_oracle(oracle, case, mode):
+    (k, rate, ebn0), (G, corr, r_max) = case, mode
     code = codes.ar4ja(k, rate)
     mu, sg = channel_params(ebn0, code.k / code.n_tx)
     n = {1024: 128, 4096: 24, 16384: 6}[k]
@@ -43,16 +48,18 @@ def test_decode_matches_oracle(oracle, k, rate, ebn0, G, corr, r_max):
 
 
 @pytest.mark.parametrize("k,rate,ebn0", CASES[:2])
-def test_monte_carlo_matches_oracle(oracle, k, rate, ebn0):
+@pytest.mark.parametrize("corr", [0, 2])
+def test_monte_carlo_matches_oracle(oracle, k, rate, ebn0, corr):
     code = codes.ar4ja(k, rate)
     mu, sg = channel_params(ebn0, code.k / code.n_tx)
     n = 1536
     a, b = params(code, 1, 3)
+    print(f"Profile D Monte Carlo K={k} rate={rate} corr={corr}", flush=True)
     with decoder.Engine(k, rate, max_iter=30, profile="D") as eng:
-        eng.set_dmin_params(a, b)
+        eng.set_dmin_params(a, b, corr)
         got = eng.run_point(ebn0, n, seed=17, first_cw=9, chunk_cw=n)
     llr = oracle.gen_channel(code, 17, 9, n, mu, sg)
-    exp = DO.run_point_d(code, llr, a, b, 30)
+    exp = DO.run_point_d(code, llr, a, b, 30, corr=corr)
     assert (got.blocks, got.frame_errors, got.bit_errors, got.iter_sum) == \
         (exp["blocks"], exp["frame_errors"], exp["bit_errors"], exp["iter_sum"])
     np.testing.assert_array_equal(got.hist, exp["hist"])
@@ -108,6 +115,22 @@ def test_monte_carlo_k16384_queue_agrees_with_decode():
 def test_same_channel_as_profile_t():
     with decoder.Engine(1024, "1/2", profile="D", max_iter=20) as d, decoder.Engine(1024, "1/2") as t:
         np.testing.assert_array_equal(d.gen_channel(3, 100, 4, 1.5), t.gen_channel(3, 100, 4, 1.5))
+
+
+def test_three_minima_rejected_on_k16384_r12():
+    """K=16384 r1/2's kernel keeps the two-minimum message form only: the
+    three-minimum correction is refused with LDPC_ERR_UNSUPPORTED, and the
+    other codes and modes are accepted."""
+    from ldpc_mi355x import _lib as L
+    code = codes.ar4ja(16384, "1/2")
+    a, b = _params_k16384(code)
+    with decoder.Engine(16384, "1/2", profile="D", max_iter=20) as eng:
+        with pytest.raises(L.LdpcError) as e:
+            eng.set_dmin_params(a, b, 2)
+        assert e.value.status == L.LDPC_ERR_UNSUPPORTED
+        eng.set_dmin_params(a, b, 1)
+        with pytest.raises(L.LdpcError):
+            eng.set_dmin_params(a, b, 3)
 
 
 def test_parameters_required():

@@ -611,6 +611,9 @@ hipError_t attributes(hipFuncAttributes *a) {
 //   q = L - R; (min1, idx1), (min2, idx2) first-index; g = min2 on idx1,
 //   min1 (or max(min1 - corr(min2 - min1), 0)) elsewhere; z = alpha g - beta;
 //   r = copysign(min(max(z, 0), r_max), s ^ sign q); L = q + r; R = r.
+//   With three minima (correction 2), g = min*(min2, min3) on idx1,
+//   min*(min1, min3) on idx2 and min*(min1, min2) elsewhere, min*(lo, hi) =
+//   max(lo - corr(hi - lo), 0).
 // Hard decision L < 0, bit errors L <= 0, as in Profile T.
 // ---------------------------------------------------------------------------
 template <int ORD_>
@@ -632,6 +635,59 @@ struct CfgD {
     static constexpr int NR = r_off(NLAYERS);  // messages per thread = parameters per row
     static_assert(G >= 16 && G <= 1024, "one check per thread");
 };
+
+// min*(lo, hi) of two magnitudes lo <= hi, Approx-Min* correction lines
+__device__ __forceinline__ float amin_d(float lo, float hi) {
+    const float C1 = 0.69140625f, K1 = -0.3984375f, C2 = 0.65625f, K2 = -0.326171875f;
+    const float x = __fsub_rn(hi, lo);
+    const float l1 = __fadd_rn(__fmul_rn(K1, x), C1);
+    const float l2 = __fadd_rn(__fmul_rn(K2, x), C2);
+    const float lm = l1 >= l2 ? l1 : l2;
+    const float o = __fsub_rn(lo, lm > 0.0f ? lm : 0.0f);
+    return o > 0.0f ? o : 0.0f;
+}
+
+// One layer with the three-minimum correction.
+template <class CF, int LI>
+__device__ __forceinline__ void run_layer_d3(float *Ls, float (&R)[CF::NR], int t, const float *al, const float *be,
+                                             float rcap) {
+    constexpr ar4ja::Layer Ly = CF::S.layer[LI];
+    constexpr int D = Ly.deg, R0 = CF::r_off(LI), Z = CF::Z;
+    asm volatile("" : "+v"(t));
+    typedef const __attribute__((address_space(4))) float cfloat;
+    const cfloat *alc = (const cfloat *)al, *bec = (const cfloat *)be;
+    int v[D];
+    float q[D];
+    float m1 = __builtin_inff(), m2 = __builtin_inff(), m3 = __builtin_inff();
+    int i1 = 0, i2 = 0;
+    uint32_t s = 0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        v[d] = Ly.slot[d].base + ((t + Ly.slot[d].shift) & (Z - 1));
+        q[d] = __fsub_rn(Ls[v[d]], R[R0 + d]);
+        s ^= f2u(q[d]);
+        // the three smallest |q| in order, first index on ties: with
+        // m1 <= m2 <= m3 the new m3 is med3(a, m2, m3) and the new m2
+        // med3(a, m1, m2); idx2 becomes the old idx1 if a < m1, d if
+        // m1 <= a < m2
+        const float a = fabsf(q[d]);
+        i2 = a < m1 ? i1 : (a < m2 ? d : i2);
+        i1 = a < m1 ? d : i1;
+        m3 = __builtin_amdgcn_fmed3f(a, m2, m3);
+        m2 = __builtin_amdgcn_fmed3f(a, m1, m2);
+        m1 = fminf(a, m1);
+    }
+    const float o12 = amin_d(m1, m2), o13 = amin_d(m1, m3), o23 = amin_d(m2, m3);
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        const float g = d == i1 ? o23 : (d == i2 ? o13 : o12);
+        const float z = __fsub_rn(__fmul_rn(alc[R0 + d], g), bec[R0 + d]);
+        const float mag = fminf(z > 0.0f ? z : 0.0f, rcap);
+        const float r = u2f(f2u(mag) | ((s ^ f2u(q[d])) & 0x80000000u));
+        Ls[v[d]] = __fadd_rn(q[d], r);
+        R[R0 + d] = r;
+    }
+}
 
 template <class CF, int LI>
 __device__ __forceinline__ void run_layer_d(float *Ls, float (&R)[CF::NR], int t, const float *al, const float *be,
@@ -684,13 +740,24 @@ __device__ __forceinline__ void run_layer_d(float *Ls, float (&R)[CF::NR], int t
     }
 }
 
-template <class CF, int... LI>
+// THREE: the three-minimum correction (its own kernel instance: a branch
+// between the two forms inside one kernel raised the register count of both,
+// 109 -> 162 VGPRs for K=1024 r1/2); otherwise corr selects the two-minimum
+// correction or plain min-sum.
+template <class CF, bool THREE, int... LI>
 __device__ __forceinline__ void run_iteration_d(float *Ls, float (&R)[CF::NR], int t, const float *al, const float *be,
                                                 bool corr, float rcap, std::integer_sequence<int, LI...>) {
-    if constexpr (CF::WAVE)
-        ((run_layer_d<CF, LI>(Ls, R, t, al, be, corr, rcap), wave_fence()), ...);
-    else
-        ((run_layer_d<CF, LI>(Ls, R, t, al, be, corr, rcap), __syncthreads()), ...);
+    if constexpr (THREE) {
+        if constexpr (CF::WAVE)
+            ((run_layer_d3<CF, LI>(Ls, R, t, al, be, rcap), wave_fence()), ...);
+        else
+            ((run_layer_d3<CF, LI>(Ls, R, t, al, be, rcap), __syncthreads()), ...);
+    } else {
+        if constexpr (CF::WAVE)
+            ((run_layer_d<CF, LI>(Ls, R, t, al, be, corr, rcap), wave_fence()), ...);
+        else
+            ((run_layer_d<CF, LI>(Ls, R, t, al, be, corr, rcap), __syncthreads()), ...);
+    }
 }
 
 template <class CF, int LI>
@@ -767,7 +834,7 @@ __device__ __forceinline__ int d_row(const MultiParams &P, int it) {
     return P.d_groups <= 1 ? 0 : (it < P.d_groups ? it : P.d_groups - 1);
 }
 
-template <class CF>
+template <class CF, bool THREE>
 __global__ void __launch_bounds__(CF::THREADS) k_d_wave(MultiParams P) {
     __shared__ __attribute__((aligned(16))) float smem[CF::CW_PER_WG * CF::N];
     __shared__ uint32_t hist_s[HIST_LDS];
@@ -829,7 +896,7 @@ __global__ void __launch_bounds__(CF::THREADS) k_d_wave(MultiParams P) {
         }
         if (!active) break;
         const int row = d_row(P, it);
-        run_iteration_d<CF>(Ls, R, t, P.d_alpha + row * CF::NR, P.d_beta + row * CF::NR, corr, rcap,
+        run_iteration_d<CF, THREE>(Ls, R, t, P.d_alpha + row * CF::NR, P.d_beta + row * CF::NR, corr, rcap,
                             std::make_integer_sequence<int, NLAYERS>{});
         it += 1;
         conv = syndrome_pass_d<CF, 0>(Ls, t, active);
@@ -844,7 +911,7 @@ __global__ void __launch_bounds__(CF::THREADS) k_d_wave(MultiParams P) {
     hist_flush(hist_s, P, CF::THREADS);
 }
 
-template <class CF>
+template <class CF, bool THREE>
 __global__ void __launch_bounds__(CF::THREADS) k_d_group(MultiParams P) {
     __shared__ __attribute__((aligned(16))) float Ls[CF::N];
     __shared__ uint32_t hist_s[HIST_LDS];
@@ -880,7 +947,7 @@ __global__ void __launch_bounds__(CF::THREADS) k_d_group(MultiParams P) {
         bool ok = false;
         while (it < P.max_iter) {
             const int row = d_row(P, it);
-            run_iteration_d<CF>(Ls, R, t, P.d_alpha + row * CF::NR, P.d_beta + row * CF::NR, corr, rcap,
+            run_iteration_d<CF, THREE>(Ls, R, t, P.d_alpha + row * CF::NR, P.d_beta + row * CF::NR, corr, rcap,
                                 std::make_integer_sequence<int, NLAYERS>{});
             ++it;
             scan_fails_from_d<CF, 0>(Ls, t, fail_s + (it & 1));
@@ -1292,17 +1359,24 @@ hipError_t attributes_db(hipFuncAttributes *a) {
 }
 #endif
 
-template <class CF>
+template <class CF, bool THREE = false>
 constexpr auto kernel_of_d() {
     if constexpr (CF::WAVE)
-        return &k_d_wave<CF>;
+        return &k_d_wave<CF, THREE>;
     else
-        return &k_d_group<CF>;
+        return &k_d_group<CF, THREE>;
 }
 
+// The grid is sized by the residency of the plain / two-minimum instance;
+// where the three-minimum instance holds fewer workgroups per CU, the ones
+// beyond its residency start as others finish and leave at their first queue
+// draw once the queue is empty (no workgroup waits on another).
 template <class CF>
 hipError_t launch_d(const MultiParams &p, int grid, hipStream_t st) {
-    hipLaunchKernelGGL(kernel_of_d<CF>(), dim3(grid), dim3(CF::THREADS), 0, st, p);
+    if (p.d_corr == 2)
+        hipLaunchKernelGGL((kernel_of_d<CF, true>()), dim3(grid), dim3(CF::THREADS), 0, st, p);
+    else
+        hipLaunchKernelGGL(kernel_of_d<CF>(), dim3(grid), dim3(CF::THREADS), 0, st, p);
     return hipGetLastError();
 }
 

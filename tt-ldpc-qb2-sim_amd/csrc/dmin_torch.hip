@@ -4,10 +4,13 @@
 //
 // Decoder (AR4JA layered schedule, no early exit, T iterations, fp32):
 //   q_d = L_v(d) - R_d;  a_d = |q_d|;  (min1, idx1), (min2, idx2) first-index
-//   g_d = min2 on idx1, min1 elsewhere (min-sum) or max(min1 - corr(min2 -
-//         min1), 0) elsewhere with the Approx-Min* correction
+//   g_d = min2 on idx1, min1 elsewhere (min-sum, corr 0), or with the
+//         Approx-Min* correction min*(a, b) = max(a - corr(b - a), 0),
 //         corr(x) = max(C1 + K1 x, C2 + K2 x, 0) (bf16-truncated constants of
-//         the reference, sfpi_host_emu.h:566-584)
+//         the reference, sfpi_host_emu.h:566-584):
+//         corr 1: min2 on idx1, min*(min1, min2) elsewhere;
+//         corr 2 (three minima): min*(min2, min3) on idx1, min*(min1, min3)
+//         on idx2, min*(min1, min2) elsewhere
 //   z_d = alpha_p * g_d - beta_p;  r_d = copysign(max(z_d, 0), s ^ sign q_d),
 //   the magnitude capped at r_max if r_max > 0
 //   L_v(d) = q_d + r_d;  R_d = r_d
@@ -28,7 +31,7 @@
 // relu, the cap, the correction's max and the clamp at zero pass the gradient
 // only through the selected branch (0 < z < r_max; line 1 if l1 >= l2;
 // corrected min > 0);
-// min / second-min route it to idx1 / idx2.
+// the minima route it to their edges (idx1 / idx2 / idx3).
 #include <c10/hip/HIPStream.h>
 #include <hip/hip_runtime.h>
 #include <torch/extension.h>
@@ -51,57 +54,92 @@ struct LayerArgs {
 __device__ __forceinline__ uint32_t f2u(float f) { return __float_as_uint(f); }
 __device__ __forceinline__ float u2f(uint32_t u) { return __uint_as_float(u); }
 
-// Check-node core shared by forward and backward.
-template <int D, bool CORR>
+// min*(lo, hi) of two magnitudes lo <= hi with the Approx-Min* correction:
+// max(lo - corr(hi - lo), 0).  cp is d corr / dx of the selected line (0 if
+// corr == 0); pos records lo - corr > 0 (the gradient passes only then).
+struct Pair {
+    float o, cp;
+    bool pos;
+};
+__device__ __forceinline__ Pair amin_pair(float lo, float hi) {
+    const float x = __fsub_rn(hi, lo);
+    const float l1 = __fadd_rn(__fmul_rn(K1, x), C1);
+    const float l2 = __fadd_rn(__fmul_rn(K2, x), C2);
+    const float lm = l1 >= l2 ? l1 : l2;
+    const float c = lm > 0.0f ? lm : 0.0f;
+    Pair p;
+    p.cp = lm > 0.0f ? (l1 >= l2 ? K1 : K2) : 0.0f;
+    const float o = __fsub_rn(lo, c);
+    p.pos = o > 0.0f;
+    p.o = p.pos ? o : 0.0f;
+    return p;
+}
+
+// Check-node core shared by forward and backward.  CM selects the
+// magnitude g_d: 0 min-sum (min2 on idx1, min1 elsewhere); 1 the two-minimum
+// correction (min2 on idx1, min*(min1, min2) elsewhere); 2 the
+// three-minimum correction, each edge's pair taken from the others:
+// min*(min2, min3) on idx1, min*(min1, min3) on idx2, min*(min1, min2)
+// elsewhere.
+template <int D, int CM>
 struct Check {
     float q[D];
     int v[D];
-    float m1, m2;
-    int i1, i2;
-    uint32_t s;      // XOR of the sign bits
-    float other;     // g on every edge but idx1
-    float cp;        // d corr / d x of the selected line (0 if corr == 0)
-    bool other_pos;  // min1 - corr > 0 (correction mode)
+    float m1, m2, m3;
+    int i1, i2, i3;
+    uint32_t s;  // XOR of the sign bits
+    Pair p12, p13, p23;
 
     __device__ __forceinline__ void minima() {
         m1 = __builtin_inff();
         m2 = __builtin_inff();
+        m3 = __builtin_inff();
         i1 = 0;
         i2 = 0;
+        i3 = 0;
         s = 0;
 #pragma unroll
         for (int d = 0; d < D; ++d) {
             s ^= f2u(q[d]) >> 31;
             const float a = fabsf(q[d]);
             if (a < m1) {
+                if constexpr (CM == 2) {
+                    m3 = m2;
+                    i3 = i2;
+                }
                 m2 = m1;
                 i2 = i1;
                 m1 = a;
                 i1 = d;
             } else if (a < m2) {
+                if constexpr (CM == 2) {
+                    m3 = m2;
+                    i3 = i2;
+                }
                 m2 = a;
                 i2 = d;
+            } else if (CM == 2 && a < m3) {
+                m3 = a;
+                i3 = d;
             }
         }
-        other = m1;
-        cp = 0.0f;
-        other_pos = true;
-        if constexpr (CORR) {
-            const float x = __fsub_rn(m2, m1);
-            const float l1 = __fadd_rn(__fmul_rn(K1, x), C1);
-            const float l2 = __fadd_rn(__fmul_rn(K2, x), C2);
-            const float lm = l1 >= l2 ? l1 : l2;
-            const float c = lm > 0.0f ? lm : 0.0f;
-            cp = lm > 0.0f ? (l1 >= l2 ? K1 : K2) : 0.0f;
-            const float o = __fsub_rn(m1, c);
-            other_pos = o > 0.0f;
-            other = other_pos ? o : 0.0f;
+        if constexpr (CM == 0) {
+            p12 = Pair{m1, 0.0f, true};
+        } else {
+            p12 = amin_pair(m1, m2);
+        }
+        if constexpr (CM == 2) {
+            p13 = amin_pair(m1, m3);
+            p23 = amin_pair(m2, m3);
         }
     }
-    __device__ __forceinline__ float g(int d) const { return d == i1 ? m2 : other; }
+    __device__ __forceinline__ float g(int d) const {
+        if constexpr (CM == 2) return d == i1 ? p23.o : (d == i2 ? p13.o : p12.o);
+        return d == i1 ? m2 : p12.o;
+    }
 };
 
-template <int D, bool CORR>
+template <int D, int CM>
 __global__ void __launch_bounds__(TPB) k_fwd(float *__restrict__ L, float *__restrict__ R, float *__restrict__ tape,
                                              const float *__restrict__ alpha, const float *__restrict__ beta,
                                              LayerArgs la, float r_max, int64_t n_threads) {
@@ -111,7 +149,7 @@ __global__ void __launch_bounds__(TPB) k_fwd(float *__restrict__ L, float *__res
     const int j = (int)(gid % la.z);
     float *Lc = L + cw * la.n;
     float *Rc = R + cw * la.e + la.e_off;
-    Check<D, CORR> c;
+    Check<D, CM> c;
 #pragma unroll
     for (int d = 0; d < D; ++d) {
         c.v[d] = la.base[d] + ((j + la.shift[d]) & (la.z - 1));
@@ -137,7 +175,7 @@ __device__ __forceinline__ float wave_sum(float v) {
     return v;
 }
 
-template <int D, bool CORR>
+template <int D, int CM>
 __global__ void __launch_bounds__(TPB) k_bwd(float *__restrict__ gL, float *__restrict__ gR,
                                              const float *__restrict__ tape, const float *__restrict__ alpha,
                                              const float *__restrict__ beta, float *__restrict__ partial,
@@ -153,7 +191,7 @@ __global__ void __launch_bounds__(TPB) k_bwd(float *__restrict__ gL, float *__re
         const int j = (int)(gid % la.z);
         float *gLc = gL + cw * la.n;
         float *gRc = gR + cw * la.e + la.e_off;
-        Check<D, CORR> c;
+        Check<D, CM> c;
 #pragma unroll
         for (int d = 0; d < D; ++d) {
             c.v[d] = la.base[d] + ((j + la.shift[d]) & (la.z - 1));
@@ -176,11 +214,20 @@ __global__ void __launch_bounds__(TPB) k_bwd(float *__restrict__ gL, float *__re
             pb[d] = -gz;
             const float gg = gz * alpha[la.p_off + d];
             dq[d] = gln;  // L' = q + r
-            if (d == c.i1) {
+            if constexpr (CM == 2) {
+                // the pair of the edge's g: (lo, hi) minima and their correction
+                const bool e1 = d == c.i1, e2 = d == c.i2;
+                const int lo = e1 ? c.i2 : c.i1, hi = (e1 || e2) ? c.i3 : c.i2;
+                const Pair &pr = e1 ? c.p23 : (e2 ? c.p13 : c.p12);
+                if (pr.pos) {
+                    ga[lo] += gg * (1.0f + pr.cp);
+                    ga[hi] += gg * -pr.cp;
+                }
+            } else if (d == c.i1) {
                 ga[c.i2] += gg;
-            } else if (c.other_pos) {
-                ga[c.i1] += gg * (1.0f + c.cp);
-                ga[c.i2] += gg * -c.cp;
+            } else if (c.p12.pos) {
+                ga[c.i1] += gg * (1.0f + c.p12.cp);
+                ga[c.i2] += gg * -c.p12.cp;
             }
         }
 #pragma unroll
@@ -250,39 +297,39 @@ Sched make_sched(const at::Tensor &tab, int64_t n) {
     return s;
 }
 
-template <int D, bool CORR>
+template <int D, int CM>
 void fwd_launch(float *L, float *R, float *tape, const float *al, const float *be, const LayerArgs &la, float r_max,
                 int64_t B, hipStream_t st) {
     const int64_t nt = B * la.z;
-    hipLaunchKernelGGL((k_fwd<D, CORR>), dim3((unsigned)((nt + TPB - 1) / TPB)), dim3(TPB), 0, st, L, R, tape, al, be,
+    hipLaunchKernelGGL((k_fwd<D, CM>), dim3((unsigned)((nt + TPB - 1) / TPB)), dim3(TPB), 0, st, L, R, tape, al, be,
                        la, r_max, nt);
 }
 
-template <int D, bool CORR>
+template <int D, int CM>
 void bwd_launch(float *gL, float *gR, const float *tape, const float *al, const float *be, float *partial,
                 const LayerArgs &la, float r_max, int64_t B, hipStream_t st) {
     const int64_t nt = B * la.z;
-    hipLaunchKernelGGL((k_bwd<D, CORR>), dim3((unsigned)((nt + TPB - 1) / TPB)), dim3(TPB), 0, st, gL, gR, tape, al,
+    hipLaunchKernelGGL((k_bwd<D, CM>), dim3((unsigned)((nt + TPB - 1) / TPB)), dim3(TPB), 0, st, gL, gR, tape, al,
                        be, partial, la, r_max, nt);
 }
 
-#define LDPC_DISPATCH(deg, corr, F, ...)                 \
-    do {                                                 \
-        if (corr) {                                      \
-            switch (deg) {                               \
-                case 3: F<3, true>(__VA_ARGS__); break;  \
-                case 6: F<6, true>(__VA_ARGS__); break;  \
-                case 10: F<10, true>(__VA_ARGS__); break; \
-                default: F<18, true>(__VA_ARGS__); break; \
-            }                                            \
-        } else {                                         \
-            switch (deg) {                               \
-                case 3: F<3, false>(__VA_ARGS__); break; \
-                case 6: F<6, false>(__VA_ARGS__); break; \
-                case 10: F<10, false>(__VA_ARGS__); break; \
-                default: F<18, false>(__VA_ARGS__); break; \
-            }                                            \
-        }                                                \
+#define LDPC_DISPATCH_DEG(deg, cm, F, ...)              \
+    do {                                               \
+        switch (deg) {                                 \
+            case 3: F<3, cm>(__VA_ARGS__); break;      \
+            case 6: F<6, cm>(__VA_ARGS__); break;      \
+            case 10: F<10, cm>(__VA_ARGS__); break;    \
+            default: F<18, cm>(__VA_ARGS__); break;    \
+        }                                              \
+    } while (0)
+#define LDPC_DISPATCH(deg, corr, F, ...)                               \
+    do {                                                               \
+        if (corr == 2)                                                 \
+            LDPC_DISPATCH_DEG(deg, 2, F, __VA_ARGS__);                 \
+        else if (corr == 1)                                            \
+            LDPC_DISPATCH_DEG(deg, 1, F, __VA_ARGS__);                 \
+        else                                                           \
+            LDPC_DISPATCH_DEG(deg, 0, F, __VA_ARGS__);                 \
     } while (0)
 
 void check_inputs(const at::Tensor &llr, const at::Tensor &alpha, const at::Tensor &beta, const Sched &s,
@@ -298,13 +345,16 @@ void check_inputs(const at::Tensor &llr, const at::Tensor &alpha, const at::Tens
     TORCH_CHECK(T >= 1, "T >= 1");
 }
 
+void check_corr(int64_t corr) { TORCH_CHECK(corr >= 0 && corr <= 2, "corr: 0 (min-sum), 1 or 2 (minima corrected)"); }
+
 // Returns [out, ckL_0, ckR_0, ckL_1, ckR_1, ...] (checkpoints only if keep);
 // out is L after the last iteration [B, N], or after every iteration
 // [T, B, N] if all_iters.
 std::vector<at::Tensor> dmin_forward(at::Tensor llr, at::Tensor alpha, at::Tensor beta, at::Tensor tab, int64_t T,
-                                     int64_t ckpt_every, bool corr, bool keep, double r_max, bool all_iters) {
+                                     int64_t ckpt_every, int64_t corr, bool keep, double r_max, bool all_iters) {
     const Sched s = make_sched(tab, llr.size(1));
     check_inputs(llr, alpha, beta, s, T);
+    check_corr(corr);
     TORCH_CHECK(ckpt_every >= 1, "ckpt_every >= 1");
     const int64_t B = llr.size(0);
     hipStream_t st = c10::hip::getCurrentHIPStream(llr.device().index()).stream();
@@ -335,13 +385,14 @@ std::vector<at::Tensor> dmin_forward(at::Tensor llr, at::Tensor alpha, at::Tenso
 // window of trunc iterations only sees the losses of its own iterations.
 // Returns [grad_llr, grad_alpha, grad_beta].
 std::vector<at::Tensor> dmin_backward(at::Tensor grad_out, std::vector<at::Tensor> ckpts, at::Tensor alpha,
-                                      at::Tensor beta, at::Tensor tab, int64_t T, int64_t ckpt_every, bool corr,
+                                      at::Tensor beta, at::Tensor tab, int64_t T, int64_t ckpt_every, int64_t corr,
                                       double r_max, int64_t trunc) {
     const bool all_iters = grad_out.dim() == 3;
     TORCH_CHECK(!all_iters || grad_out.size(0) == T, "per-iteration gradient must be [T, B, N]");
     const at::Tensor g_last = all_iters ? grad_out[T - 1] : grad_out;
     const Sched s = make_sched(tab, g_last.size(1));
     check_inputs(g_last.contiguous(), alpha, beta, s, T);
+    check_corr(corr);
     const int nl = (int)s.layers.size();
     const int64_t n_steps = T * nl, n_seg = (n_steps + ckpt_every - 1) / ckpt_every;
     TORCH_CHECK((int64_t)ckpts.size() == 2 * n_seg, "checkpoint list does not match T and ckpt_every");

@@ -39,7 +39,7 @@ def run_forward(z, ebn0, blocks, seed, target_fe, chunk=1024, T=0):
     k, rate, T = int(z["k_info"]), str(z["rate"]), (T or int(z["T"]))
     if z["alpha"].shape[0] > 1 and T != int(z["T"]):
         raise SystemExit("--forward-T needs shared (not per-iteration) parameters")
-    model = dmin.DMinSum(k, rate, T=T, correction=bool(z["correction"]), r_max=float(z["r_max"]),
+    model = dmin.DMinSum(k, rate, T=T, correction=int(z["correction"]), r_max=float(z["r_max"]),
                          per_iteration=z["alpha"].shape[0] > 1)
     with torch.no_grad():
         model.alpha.copy_(torch.from_numpy(z["alpha"]))
@@ -88,6 +88,7 @@ def main(argv=None):
     ap.add_argument("--forward", action="store_true",
                     help="decode the trained decoder with its training forward (T iterations, no early exit)")
     ap.add_argument("--forward-T", type=int, default=0, help="iterations of the --forward decode (0: the trained T)")
+    ap.add_argument("--forward-chunk", type=int, default=1024, help="codewords per --forward decode call")
     ap.add_argument("--segment", type=int, default=1 << 24,
                     help="codewords per progress line (a point runs as consecutive segments of codeword ids)")
     ap.add_argument("--decoders", nargs="+", choices=["trained", "approx_min_star"],
@@ -102,7 +103,8 @@ def main(argv=None):
         pts = []
         t_fwd = args.forward_T or int(z["T"])
         for eb in args.ebn0:
-            done, fe, be, sec, n_tx = run_forward(z, eb, args.blocks, args.seed, args.target_fe, T=t_fwd)
+            done, fe, be, sec, n_tx = run_forward(z, eb, args.blocks, args.seed, args.target_fe,
+                                                  chunk=args.forward_chunk, T=t_fwd)
             lo, hi = poisson_ci95(fe)
             pts.append((eb, fe / done))
             print(json.dumps({"decoder": f"trained_forward_T{t_fwd}", "max_iter": t_fwd, "ebn0_db": eb,
@@ -120,7 +122,7 @@ def main(argv=None):
             eng = decoder.Engine(k, rate, max_iter=max_iter)
         else:
             eng = decoder.Engine(k, rate, max_iter=max_iter, profile="D")
-            eng.set_dmin_params(z["alpha"], z["beta"], bool(z["correction"]), float(z["r_max"]))
+            eng.set_dmin_params(z["alpha"], z["beta"], int(z["correction"]), float(z["r_max"]))
         pts = []
         for eb in args.ebn0:
             r = None

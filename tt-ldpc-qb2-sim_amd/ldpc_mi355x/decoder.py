@@ -275,8 +275,9 @@ class Engine:
         self._check(min(n, 0))
         return ids[:n], int(seen.value)
 
-    def set_dmin_params(self, alpha, beta, correction: bool = False, r_max: float = 0.0) -> None:
-        """Profile D scale / offset rows [groups, n_params] (float32)."""
+    def set_dmin_params(self, alpha, beta, correction: int = 0, r_max: float = 0.0) -> None:
+        """Profile D scale / offset rows [groups, n_params] (float32);
+        correction 0 / False min-sum, 1 / True the two-minimum correction."""
         a = np.ascontiguousarray(alpha, dtype=np.float32)
         b = np.ascontiguousarray(beta, dtype=np.float32)
         if a.ndim == 1:
@@ -284,7 +285,7 @@ class Engine:
         if a.shape != b.shape:
             raise ValueError("alpha and beta must have the same shape")
         self._check(self._lib.ldpc_set_dmin_params(self._h, a.ctypes.data, b.ctypes.data, a.shape[0], a.shape[1],
-                                                   int(bool(correction)), float(r_max)))
+                                                   int(correction), float(r_max)))
 
     def schedule(self):
         order = L.variant_fields(self.variant)[1] if self.variant else 0

@@ -10,7 +10,10 @@ iterations or one row per iteration.  The check update is
 
 with g_d the leave-one-out minimum (second minimum on the argmin edge),
 optionally with the Approx-Min* correction of the reference's golden tree
-(bf16-truncated constants, sfpi_host_emu.h:566-584).  Forward and backward
+(bf16-truncated constants, sfpi_host_emu.h:566-584): correction 1 corrects
+the minimum the other edges get with the second; correction 2 keeps three
+minima and gives every edge min* of the two smallest magnitudes among the
+others (min*(min2, min3) on the argmin edge, min*(min1, min3) on the second).  Forward and backward
 are hand-written HIP kernels (csrc/dmin_torch.hip) behind a
 ``torch.autograd.Function``; activations are recomputed from (L, R)
 checkpoints taken every ``ckpt_every`` layer steps.  The reference has no
@@ -61,6 +64,7 @@ class DMinSumFunction(torch.autograd.Function):
     @staticmethod
     def forward(ctx, llr, alpha, beta, tab, T, ckpt_every, corr, r_max=0.0, all_iters=False, trunc=0):
         need = ctx.needs_input_grad[0] or ctx.needs_input_grad[1] or ctx.needs_input_grad[2]
+        corr = int(corr)
         outs = _ext.forward(llr.contiguous(), alpha.contiguous(), beta.contiguous(), tab, T, ckpt_every, corr, need,
                             float(r_max), bool(all_iters))
         ctx.ckpts = outs[1:]
@@ -82,9 +86,14 @@ class DMinSum(torch.nn.Module):
     """Learnable layered min-sum decoder for one AR4JA code."""
 
     def __init__(self, k_info: int = 1024, rate: str = "1/2", T: int = 20, ckpt_every: int = 4,
-                 correction: bool = False, per_iteration: bool = False, r_max: float = 0.0, trunc: int = 0,
+                 correction: int = 0, per_iteration: bool = False, r_max: float = 0.0, trunc: int = 0,
                  device="cuda"):
+        """correction: 0 (or False) plain min-sum, 1 (or True) the two-minimum
+        Approx-Min* correction, 2 the three-minimum one."""
         super().__init__()
+        correction = int(correction)
+        if correction not in (0, 1, 2):
+            raise ValueError("correction must be 0, 1 or 2")
         self.code = codes.ar4ja(k_info, rate)
         self.k_info, self.rate, self.T = k_info, rate, T
         self.ckpt_every, self.correction, self.r_max, self.trunc = ckpt_every, correction, r_max, trunc
@@ -103,7 +112,14 @@ class DMinSum(torch.nn.Module):
         return self.code.n_tx
 
     def forward(self, llr: torch.Tensor, all_iters: bool = False) -> torch.Tensor:
-        """Soft output after T iterations [B, N], or after each [T, B, N]."""
+        """Soft output after T iterations [B, N], or after each [T, B, N].
+        With gradients disabled (torch.no_grad) no checkpoints are kept: the
+        autograd.Function would keep them whenever the parameters require
+        grad, T * 12 / ckpt_every copies of (L, R)."""
+        if not torch.is_grad_enabled():
+            return _ext.forward(llr.contiguous(), self.alpha.detach().contiguous(), self.beta.detach().contiguous(),
+                                self.tab, self.T, self.ckpt_every, int(self.correction), False, float(self.r_max),
+                                bool(all_iters))[0]
         return DMinSumFunction.apply(llr, self.alpha, self.beta, self.tab, self.T, self.ckpt_every,
                                      self.correction, self.r_max, all_iters, self.trunc)
 

@@ -60,7 +60,7 @@ def mine_hard(model, args, step, hard, mined):
     p = model.export()
     first = (step + 1) * args.hard_pool  # a new pool of codeword ids each round
     with decoder.Engine(args.k, args.rate, max_iter=args.hard_iters, profile="D") as eng:
-        eng.set_dmin_params(p["alpha"], p["beta"], bool(p["correction"]), float(p["r_max"]))
+        eng.set_dmin_params(p["alpha"], p["beta"], int(p["correction"]), float(p["r_max"]))
         eng.set_capture(args.hard_keep)
         r = eng.run_point(args.hard_ebn0, args.hard_pool, seed=MINE_SEED, first_cw=first, chunk_cw=1 << 20)
         ids, seen = eng.captures(args.hard_keep)
@@ -89,7 +89,9 @@ def main(argv=None):
     ap.add_argument("--ebn0-hi", type=float, default=2.5)
     ap.add_argument("--ckpt-every", type=int, default=4)
     ap.add_argument("--r-max", type=float, default=0.0, help="cap on check-message magnitude (0: none)")
-    ap.add_argument("--correction", action="store_true", help="Approx-Min* correction on the min-sum magnitude")
+    ap.add_argument("--correction", type=int, nargs="?", const=1, default=0, choices=[0, 1, 2],
+                    help="Approx-Min* correction on the min-sum magnitude: 1 (the flag alone) from the two "
+                         "smallest magnitudes, 2 from three (every edge's pair excludes the edge itself)")
     ap.add_argument("--per-iteration", action="store_true", help="one parameter row per iteration")
     ap.add_argument("--bf16-llr", action="store_true", help="round channel LLRs to bf16")
     ap.add_argument("--multi-loss", action="store_true", help="average the loss over every iteration's output")

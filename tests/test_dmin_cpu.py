@@ -130,6 +130,40 @@ def test_three_minima_give_each_edge_the_two_smallest_others(k, rate):
                 assert L[b, var[d, j]] == exp, (b, j, d)
 
 
+@pytest.mark.parametrize("k,rate", [(1024, "1/2"), (1024, "4/5")])
+def test_three_minima_select_by_magnitude_equals_select_by_index(k, rate):
+    """The Profile D kernels pick each edge's pair by its magnitude (equal to
+    min1: min*(min2, min3); equal to min2: min*(min1, min3); else
+    min*(min1, min2)) where the oracle goes by first-index argmins.  On
+    magnitudes drawn from four values (ties in nearly every check) both
+    rules give the same layer output, bit for bit."""
+    code = codes.ar4ja(k, rate)
+    sch = DO.schedule(code)
+    rng = np.random.default_rng(11)
+    B = 16
+    P = DO.n_params(code)
+    al = rng.uniform(0.8, 1.0, P).astype(np.float32)
+    be = rng.uniform(0.0, 0.2, P).astype(np.float32)
+    for li, lay in enumerate(sch[:3]):
+        deg, z, var, e_off, p_off = lay
+        llr = rng.choice(np.array([-1.5, -0.5, 0.5, 1.5, 3.0], np.float32), (B, code.n))
+        L = llr.copy()
+        R = np.zeros((B, sum(l[0] * l[1] for l in sch)), np.float32)
+        DO._layer(L, R, lay, al, be, 2, np.float32)
+        q = llr[:, var]
+        a = np.abs(q)
+        srt = np.sort(a, axis=1)
+        m1, m2, m3 = srt[:, 0:1], srt[:, 1:2], srt[:, 2:3]
+        o12, o13, o23 = DO._amin(m1, m2, np.float32), DO._amin(m1, m3, np.float32), DO._amin(m2, m3, np.float32)
+        g = np.where(a == m1, o23, np.where(a == m2, o13, o12)).astype(np.float32)
+        zz = (al[p_off:p_off + deg, None] * g).astype(np.float32) - be[p_off:p_off + deg, None]
+        mag = np.maximum(zz, np.float32(0))
+        neg = np.bitwise_xor.reduce(np.signbit(q), axis=1)[:, None, :] ^ np.signbit(q)
+        r = np.where(neg, -mag, mag).astype(np.float32)
+        assert (a == m1).sum() > (a[:, :1] >= 0).sum(), "ties present"
+        np.testing.assert_array_equal(L[:, var], (q + r).astype(np.float32), err_msg=f"layer {li}")
+
+
 def test_three_minima_on_a_degree_3_check_is_the_golden_pairwise_min_star():
     """On a degree-3 layer each edge's two smallest others are simply the two
     other edges, so corr = 2 at alpha = 1, beta = 0 equals the golden tree's

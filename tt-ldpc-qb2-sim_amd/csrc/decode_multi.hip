@@ -642,12 +642,20 @@ __device__ __forceinline__ float amin_d(float lo, float hi) {
     const float x = __fsub_rn(hi, lo);
     const float l1 = __fadd_rn(__fmul_rn(K1, x), C1);
     const float l2 = __fadd_rn(__fmul_rn(K2, x), C2);
-    const float lm = l1 >= l2 ? l1 : l2;
-    const float o = __fsub_rn(lo, lm > 0.0f ? lm : 0.0f);
+    // max(l1, l2, 0) as one v_max3 (no NaN reaches it; the lines are never -0)
+    const float o = __fsub_rn(lo, fmaxf(fmaxf(l1, l2), 0.0f));
     return o > 0.0f ? o : 0.0f;
 }
 
-// One layer with the three-minimum correction.
+// One layer with the three-minimum correction.  The edge's g is chosen by
+// its magnitude, not by the minima's indices: an edge equal to min1 gets
+// min*(min2, min3), one equal to min2 (above min1) min*(min1, min3), any
+// other min*(min1, min2).  That is the first-index rule of the oracle on
+// every input: an edge equal to min1 that is not idx1 means min1 = min2, so
+// the pair it gets by index, min*(min1, min3) (or min*(min1, min2) when it is
+// past idx2, then min3 = min1 too), equals min*(min2, min3); an edge equal to
+// min2 that is not idx2 means min2 = min3, so min*(min1, min2) =
+// min*(min1, min3).  No index is tracked.
 template <class CF, int LI>
 __device__ __forceinline__ void run_layer_d3(float *Ls, float (&R)[CF::NR], int t, const float *al, const float *be,
                                              float rcap) {
@@ -659,20 +667,15 @@ __device__ __forceinline__ void run_layer_d3(float *Ls, float (&R)[CF::NR], int 
     int v[D];
     float q[D];
     float m1 = __builtin_inff(), m2 = __builtin_inff(), m3 = __builtin_inff();
-    int i1 = 0, i2 = 0;
     uint32_t s = 0;
 #pragma unroll
     for (int d = 0; d < D; ++d) {
         v[d] = Ly.slot[d].base + ((t + Ly.slot[d].shift) & (Z - 1));
         q[d] = __fsub_rn(Ls[v[d]], R[R0 + d]);
         s ^= f2u(q[d]);
-        // the three smallest |q| in order, first index on ties: with
-        // m1 <= m2 <= m3 the new m3 is med3(a, m2, m3) and the new m2
-        // med3(a, m1, m2); idx2 becomes the old idx1 if a < m1, d if
-        // m1 <= a < m2
+        // the three smallest |q| with multiplicity: with m1 <= m2 <= m3 the
+        // new m3 is med3(a, m2, m3) and the new m2 med3(a, m1, m2)
         const float a = fabsf(q[d]);
-        i2 = a < m1 ? i1 : (a < m2 ? d : i2);
-        i1 = a < m1 ? d : i1;
         m3 = __builtin_amdgcn_fmed3f(a, m2, m3);
         m2 = __builtin_amdgcn_fmed3f(a, m1, m2);
         m1 = fminf(a, m1);
@@ -680,7 +683,8 @@ __device__ __forceinline__ void run_layer_d3(float *Ls, float (&R)[CF::NR], int 
     const float o12 = amin_d(m1, m2), o13 = amin_d(m1, m3), o23 = amin_d(m2, m3);
 #pragma unroll
     for (int d = 0; d < D; ++d) {
-        const float g = d == i1 ? o23 : (d == i2 ? o13 : o12);
+        const float a = fabsf(q[d]);
+        const float g = a == m1 ? o23 : (a == m2 ? o13 : o12);
         const float z = __fsub_rn(__fmul_rn(alc[R0 + d], g), bec[R0 + d]);
         const float mag = fminf(z > 0.0f ? z : 0.0f, rcap);
         const float r = u2f(f2u(mag) | ((s ^ f2u(q[d])) & 0x80000000u));
@@ -723,8 +727,7 @@ __device__ __forceinline__ void run_layer_d(float *Ls, float (&R)[CF::NR], int t
         const float x = __fsub_rn(m2, m1);
         const float l1 = __fadd_rn(__fmul_rn(K1, x), C1);
         const float l2 = __fadd_rn(__fmul_rn(K2, x), C2);
-        const float lm = l1 >= l2 ? l1 : l2;
-        const float o = __fsub_rn(m1, lm > 0.0f ? lm : 0.0f);
+        const float o = __fsub_rn(m1, fmaxf(fmaxf(l1, l2), 0.0f));
         other = o > 0.0f ? o : 0.0f;
     }
 #pragma unroll

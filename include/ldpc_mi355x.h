@@ -64,7 +64,7 @@ enum {
      * r_max), sign) with g the leave-one-out minimum and alpha / beta per
      * (layer, circulant) from ldpc_set_dmin_params; channel as Profile T.
      * All nine AR4JA codes; for K=16384 r1/2 ldpc_create also allocates a
-     * message workspace of 144 KiB per workgroup of the launch grid (one
+     * message workspace of 176.25 KiB per workgroup of the launch grid (one
      * workgroup per CU). */
     LDPC_PROFILE_D = 3
 };
@@ -317,9 +317,8 @@ int ldpc_get_captures(ldpc_ctx *ctx, uint64_t *cw_ids, uint32_t cap, uint64_t *n
  * = the sum of the layer degrees (n_edges / z); correction 0 is plain
  * min-sum, 1 applies the Approx-Min* correction to the minimum the other
  * edges get (from the two smallest magnitudes), 2 keeps three minima and
- * gives every edge min* of the two smallest magnitudes among the others (not
- * for K=16384 r1/2: LDPC_ERR_UNSUPPORTED); r_max > 0 caps message
- * magnitudes.  Copied to the device; required before Profile D work.  No
+ * gives every edge min* of the two smallest magnitudes among the others;
+ * r_max > 0 caps message magnitudes.  Copied to the device; required before Profile D work.  No
  * reference counterpart (the trained decoder's parameters). */
 int ldpc_set_dmin_params(ldpc_ctx *ctx, const float *alpha, const float *beta, uint32_t groups, uint32_t n_params,
                          int correction, float r_max);

@@ -24,9 +24,9 @@ def params(code, G, seed):
     return rng.uniform(0.7, 0.95, (G, P)).astype(np.float32), rng.uniform(0.0, 0.4, (G, P)).astype(np.float32)
 
 
-# the three-minimum correction (corr = 2) runs on every code but K=16384 r1/2
-DECODE_CASES = [(c, m) for c in CASES for m in [(1, False, 0.0), (8, True, 0.0), (1, False, 6.0), (8, True, 6.0)]] + \
-    [(c, m) for c in CASES if c[0] != 16384 for m in [(8, 2, 0.0), (1, 2, 6.0)]]
+# corr: 0 / False min-sum, 1 / True two-minimum correction, 2 three-minimum correction
+DECODE_CASES = [(c, m) for c in CASES for m in [(1, False, 0.0), (8, True, 0.0), (1, False, 6.0), (8, True, 6.0),
+                                                (8, 2, 0.0), (1, 2, 6.0)]]
 
 
 @pytest.mark.parametrize("case,mode", DECODE_CASES)
@@ -71,18 +71,21 @@ def _params_k16384(code):
     return rng.uniform(0.8, 0.9, (1, p)).astype(np.float32), rng.uniform(0.0, 0.1, (1, p)).astype(np.float32)
 
 
-def test_monte_carlo_k16384_matches_oracle(oracle):
+@pytest.mark.parametrize("corr,ebn0", [(1, 1.1), (2, 0.95)])
+def test_monte_carlo_k16384_matches_oracle(oracle, corr, ebn0):
     """40 codewords (all through the queue: the workgroup-wide broadcast of
-    the queued id) at 1.1 dB, converged and failed codewords."""
+    the queued id), converged and failed codewords, two- and three-minimum
+    corrections (the three-minimum kernel keeps four words per compressed
+    check)."""
     code = codes.ar4ja(16384, "1/2")
-    mu, sg = channel_params(1.1)
+    mu, sg = channel_params(ebn0)
     a, b = _params_k16384(code)
-    print("Profile D Monte Carlo K=16384 r1/2, 40 codewords", flush=True)
+    print(f"Profile D Monte Carlo K=16384 r1/2, 40 codewords, corr={corr}", flush=True)
     with decoder.Engine(16384, "1/2", max_iter=30, profile="D") as eng:
-        eng.set_dmin_params(a, b, True)
-        got = eng.run_point(1.1, 40, seed=23, first_cw=4, chunk_cw=40)
+        eng.set_dmin_params(a, b, corr)
+        got = eng.run_point(ebn0, 40, seed=23, first_cw=4, chunk_cw=40)
     llr = oracle.gen_channel(code, 23, 4, 40, mu, sg)
-    exp = DO.run_point_d(code, llr, a, b, 30, corr=True)
+    exp = DO.run_point_d(code, llr, a, b, 30, corr=corr)
     assert exp["frame_errors"] > 0 and exp["iter_sum"] < 30 * 40
     assert (got.blocks, got.frame_errors, got.bit_errors, got.iter_sum) == \
         (exp["blocks"], exp["frame_errors"], exp["bit_errors"], exp["iter_sum"])
@@ -117,20 +120,18 @@ def test_same_channel_as_profile_t():
         np.testing.assert_array_equal(d.gen_channel(3, 100, 4, 1.5), t.gen_channel(3, 100, 4, 1.5))
 
 
-def test_three_minima_rejected_on_k16384_r12():
-    """K=16384 r1/2's kernel keeps the two-minimum message form only: the
-    three-minimum correction is refused with LDPC_ERR_UNSUPPORTED, and the
-    other codes and modes are accepted."""
+def test_correction_modes_accepted():
+    """Corrections 0, 1 and 2 are accepted; anything else is an argument error."""
     from ldpc_mi355x import _lib as L
     code = codes.ar4ja(16384, "1/2")
     a, b = _params_k16384(code)
     with decoder.Engine(16384, "1/2", profile="D", max_iter=20) as eng:
-        with pytest.raises(L.LdpcError) as e:
-            eng.set_dmin_params(a, b, 2)
-        assert e.value.status == L.LDPC_ERR_UNSUPPORTED
-        eng.set_dmin_params(a, b, 1)
-        with pytest.raises(L.LdpcError):
-            eng.set_dmin_params(a, b, 3)
+        for corr in (0, 1, 2):
+            eng.set_dmin_params(a, b, corr)
+        for corr in (3, -1):
+            with pytest.raises(L.LdpcError) as e:
+                eng.set_dmin_params(a, b, corr)
+            assert e.value.status == L.LDPC_ERR_ARG
 
 
 def test_parameters_required():

@@ -1003,7 +1003,11 @@ __global__ void __launch_bounds__(CF::THREADS) k_d_group(MultiParams P) {
 // messages.  That is six words per thread and layer: six layers' words stay
 // in registers, the other six go through a per-workgroup HBM workspace laid
 // out [word][thread] (coalesced across the workgroup), 144 KiB read and
-// written per codeword-iteration.  A workspace layer's words are loaded one
+// written per codeword-iteration.  With the three-minimum correction a
+// compressed check keeps four words: g of the edges equal to min1 (min*(min2,
+// min3)), equal to min2 (min*(min1, min3)) and of the others (min*(min1,
+// min2)), and the two equality masks with the signs (176 KiB per
+// codeword-iteration; eight register words per compressed layer).  A workspace layer's words are loaded one
 // layer ahead (each word belongs to one thread and one layer, so the
 // prefetch never races a store), and iteration 0 uses R = 0 without
 // reading, so neither the registers nor the workspace are cleared.  The workgroup-wide syndrome flag and the queue broadcast
@@ -1018,8 +1022,8 @@ constexpr int db_r_off(int li) {
     return o;
 }
 // workspace words per check of layer li: the raw messages up to degree 3,
-// else the compressed form
-constexpr int db_words(int li) { return SDB.layer[li].deg <= 3 ? SDB.layer[li].deg : 3; }
+// else the compressed form (three words, four with three minima)
+constexpr int db_words(int li, bool three) { return SDB.layer[li].deg <= 3 ? SDB.layer[li].deg : (three ? 4 : 3); }
 // Layers whose words go through the workspace; the others keep theirs in
 // registers, six per thread and layer (the 128 VGPRs a 1024-thread
 // workgroup allows hold six such layers without spilling in the loop;
@@ -1028,14 +1032,14 @@ constexpr int db_words(int li) { return SDB.layer[li].deg <= 3 ? SDB.layer[li].d
 // the layer before each.
 constexpr uint32_t DB_HBM = 0xAAAu;  // layers 1, 3, 5, 7, 9, 11
 constexpr bool db_hbm(int li) { return (DB_HBM >> li) & 1u; }
-constexpr int db_w_off(int li) {
+constexpr int db_w_off(int li, bool three) {
     int o = 0;
-    for (int i = 0; i < li; ++i) o += db_hbm(i) ? db_words(i) : 0;
+    for (int i = 0; i < li; ++i) o += db_hbm(i) ? db_words(i, three) : 0;
     return o;
 }
-constexpr int db_reg_off(int li) {
+constexpr int db_reg_off(int li, bool three) {
     int o = 0;
-    for (int i = 0; i < li; ++i) o += db_hbm(i) ? 0 : 2 * db_words(i);
+    for (int i = 0; i < li; ++i) o += db_hbm(i) ? 0 : 2 * db_words(i, three);
     return o;
 }
 constexpr bool db_hbm_apart() {
@@ -1046,27 +1050,33 @@ constexpr bool db_hbm_apart() {
 static_assert(db_hbm_apart(), "workspace layers must not be adjacent");
 constexpr bool db_words_fit() {
     for (int i = 0; i < NLAYERS; ++i)
-        if (db_words(i) > 3 || SDB.layer[i].deg > 6) return false;
+        if (db_words(i, false) > 3 || db_words(i, true) > 4 || SDB.layer[i].deg > 6) return false;
     return true;
 }
-static_assert(db_words_fit(), "three words per check, degrees up to 6");
+static_assert(db_words_fit(), "three (four) words per check, degrees up to 6");
 
-struct CfgDB {
+template <bool THREE>
+struct CfgDBT {
     static constexpr int C = 2, ORD = 0;
     static constexpr ar4ja::Schedule S = SDB;
     static constexpr int Z = S.Z, N = S.N, NP = S.n_punct, NTX = S.N - S.n_punct, K = S.k_info;
     static constexpr int THREADS = 1024, G = THREADS, WPW = THREADS / 64, CW_PER_WG = 1;
     static constexpr bool WAVE = false;
     static constexpr int r_off(int li) { return db_r_off(li); }
-    static constexpr int w_off(int li) { return db_w_off(li); }
-    static constexpr int NR = db_r_off(NLAYERS);         // messages per check = parameters per row
-    static constexpr int NW = db_w_off(NLAYERS);         // workspace words per check
-    static constexpr int NREGW = db_reg_off(NLAYERS);    // register-resident words per thread
-    static constexpr int CTL = 2 * NW * THREADS;         // control words: flag[2], queue broadcast
-    static constexpr int WORK_DWORDS = CTL + 64;         // per workgroup (control padded to 256 B)
-    static constexpr int WMAX = 3;  // workspace / register words per check and layer (at most)
+    static constexpr int w_off(int li) { return db_w_off(li, THREE); }
+    static constexpr int reg_off(int li) { return db_reg_off(li, THREE); }
+    static constexpr int words(int li) { return db_words(li, THREE); }
+    static constexpr int NR = db_r_off(NLAYERS);               // messages per check = parameters per row
+    static constexpr int NW = db_w_off(NLAYERS, THREE);        // workspace words per check
+    static constexpr int NREGW = db_reg_off(NLAYERS, THREE);   // register-resident words per thread
+    static constexpr int CTL = 2 * NW * THREADS;               // control words: flag[2], queue broadcast
+    static constexpr int WORK_DWORDS = CTL + 64;               // per workgroup (control padded to 256 B)
+    static constexpr int WMAX = THREE ? 4 : 3;  // workspace / register words per check and layer (at most)
     static_assert(Z == 2 * THREADS && N * 4 == 163840, "K = 16384 r1/2 layout");
 };
+using CfgDB = CfgDBT<false>;
+// workspace stride per workgroup, the same for every instance (the larger layout)
+constexpr int DB_WORK_STRIDE = CfgDBT<true>::WORK_DWORDS;
 
 // Workgroup barrier that orders LDS only: the messages in HBM are private to
 // their thread, so a layer need not wait for its message stores (or the next
@@ -1091,9 +1101,9 @@ __device__ __forceinline__ void ctl_store(uint32_t *p, uint32_t v) {
 
 // Workspace words of layer LI for the thread's two checks: word (k, h) at
 // Rbase[(2 (W0 + k) + h) * THREADS + t].
-template <int LI>
-__device__ __forceinline__ void load_r_db(const uint32_t *Rbase, int t, uint32_t (&buf)[2 * CfgDB::WMAX]) {
-    constexpr int W = db_words(LI), W0 = CfgDB::w_off(LI);
+template <int LI, bool THREE>
+__device__ __forceinline__ void load_r_db(const uint32_t *Rbase, int t, uint32_t (&buf)[2 * CfgDBT<THREE>::WMAX]) {
+    constexpr int W = CfgDBT<THREE>::words(LI), W0 = CfgDBT<THREE>::w_off(LI);
     asm volatile("" : "+v"(t));  // addresses are formed here, per layer
     const uint32_t *Rw = Rbase + t;
 #pragma unroll
@@ -1104,9 +1114,9 @@ __device__ __forceinline__ void load_r_db(const uint32_t *Rbase, int t, uint32_t
 }
 
 // Stores of layer LI's words from buf (the layout of load_r_db).
-template <int LI>
-__device__ __forceinline__ void store_r_db(uint32_t *Rbase, int t, const uint32_t (&buf)[2 * CfgDB::WMAX]) {
-    constexpr int W = db_words(LI), W0 = CfgDB::w_off(LI);
+template <int LI, bool THREE>
+__device__ __forceinline__ void store_r_db(uint32_t *Rbase, int t, const uint32_t (&buf)[2 * CfgDBT<THREE>::WMAX]) {
+    constexpr int W = CfgDBT<THREE>::words(LI), W0 = CfgDBT<THREE>::w_off(LI);
     asm volatile("" : "+v"(t));
     uint32_t *Rw = Rbase + t;
 #pragma unroll
@@ -1198,17 +1208,83 @@ __device__ __forceinline__ void check_db(float *Ls, int j, uint32_t *w, bool fir
     }
 }
 
+// One check of layer LI with the three-minimum correction (run_layer_d3's
+// arithmetic).  Compressed words: w[h] g of the edges equal to min1, w[h + 2]
+// of those equal to min2, w[h + 4] of the others, w[h + 6] bit d: |q_d| ==
+// min1, bit 6 + d: |q_d| == min2, bit 12 + d: the message's sign.  The
+// rebuild selects g with the same equalities, so it reproduces the stored
+// messages bit for bit.
+template <int LI, int H, bool CAP>
+__device__ __forceinline__ void check_db3(float *Ls, int j, uint32_t *w, bool first, const float *pa,
+                                          const float *pb, const float *alv, const float *bev, float rcap) {
+    constexpr ar4ja::Layer Ly = CfgDB::S.layer[LI];
+    constexpr int D = Ly.deg, Z = CfgDB::Z;
+    constexpr bool PACK = D > 3;
+    static_assert(D <= 6, "packed masks hold six edges");
+    asm volatile("" : "+v"(j));
+    float r[D];
+    if constexpr (PACK) {
+        const float g1 = u2f(w[H]), g2 = u2f(w[H + 2]), go = u2f(w[H + 4]);
+        const uint32_t pk = w[H + 6];
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            const float g = ((pk >> d) & 1u) ? g1 : (((pk >> (6 + d)) & 1u) ? g2 : go);
+            const float mag = db_mag<CAP>(pa[d], pb[d], g, rcap);
+            r[d] = first ? 0.0f : u2f(f2u(mag) | (((pk >> (12 + d)) & 1u) << 31));
+        }
+    } else {
+#pragma unroll
+        for (int d = 0; d < D; ++d) r[d] = first ? 0.0f : u2f(w[H + 2 * d]);
+    }
+    int v[D];
+    float q[D];
+    float m1 = __builtin_inff(), m2 = __builtin_inff(), m3 = __builtin_inff();
+    uint32_t s = 0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        v[d] = Ly.slot[d].base + ((j + Ly.slot[d].shift) & (Z - 1));
+        q[d] = __fsub_rn(Ls[v[d]], r[d]);
+        s ^= f2u(q[d]);
+        const float a = fabsf(q[d]);
+        m3 = __builtin_amdgcn_fmed3f(a, m2, m3);
+        m2 = __builtin_amdgcn_fmed3f(a, m1, m2);
+        m1 = fminf(a, m1);
+    }
+    const float o12 = amin_d(m1, m2), o13 = amin_d(m1, m3), o23 = amin_d(m2, m3);
+    uint32_t pk = 0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        const float a = fabsf(q[d]);
+        const bool e1 = a == m1, e2 = a == m2;
+        const float mag = db_mag<CAP>(alv[d], bev[d], e1 ? o23 : (e2 ? o13 : o12), rcap);
+        const uint32_t sw = s ^ f2u(q[d]);  // bit 31: the message's sign
+        const float rr = u2f(f2u(mag) | (sw & 0x80000000u));
+        Ls[v[d]] = __fadd_rn(q[d], rr);
+        if constexpr (PACK)
+            pk |= ((uint32_t)e1 << d) | ((uint32_t)e2 << (6 + d)) | ((sw >> 31) << (12 + d));
+        else
+            w[H + 2 * d] = f2u(rr);
+    }
+    if constexpr (PACK) {
+        w[H] = f2u(o23);
+        w[H + 2] = f2u(o13);
+        w[H + 4] = f2u(o12);
+        w[H + 6] = pk;
+    }
+}
+
 // Layer LI: workspace words arrive in cur (loaded one layer earlier;
 // ignored in iteration 0): a workspace layer's in buf, loaded by the layer
 // before it, which requests them before its own compute; a register layer's
 // in Rr.  al / be are this iteration's parameter row, pal / pbe the previous
 // iteration's.
-template <int LI, bool CAP>
-__device__ __forceinline__ void layer_db(float *Ls, uint32_t *Rbase, int t, uint32_t (&buf)[2 * CfgDB::WMAX],
-                                         uint32_t (&Rr)[CfgDB::NREGW], bool first, const float *al,
+template <int LI, bool CAP, bool THREE>
+__device__ __forceinline__ void layer_db(float *Ls, uint32_t *Rbase, int t, uint32_t (&buf)[2 * CfgDBT<THREE>::WMAX],
+                                         uint32_t (&Rr)[CfgDBT<THREE>::NREGW], bool first, const float *al,
                                          const float *be, const float *pal, const float *pbe, bool corr,
                                          float rcap) {
-    constexpr int D = CfgDB::S.layer[LI].deg, R0 = CfgDB::r_off(LI), W = db_words(LI);
+    using CF = CfgDBT<THREE>;
+    constexpr int D = CF::S.layer[LI].deg, R0 = CF::r_off(LI), W = CF::words(LI);
     constexpr bool PACK = D > 3, HBM = db_hbm(LI);
     constexpr int LN = LI + 1 < NLAYERS ? LI + 1 : 0;
     // The parameter rows are read through the constant address space, i.e.
@@ -1232,32 +1308,40 @@ __device__ __forceinline__ void layer_db(float *Ls, uint32_t *Rbase, int t, uint
             pb[d] = pbec[R0 + d];
         }
     }
+    auto check = [&](int j, uint32_t *w, auto h) {
+        constexpr int HH = decltype(h)::value;
+        if constexpr (THREE)
+            check_db3<LI, HH, CAP>(Ls, j, w, first, pa, pb, alv, bev, rcap);
+        else
+            check_db<LI, HH, CAP>(Ls, j, w, first, pa, pb, alv, bev, corr, rcap);
+    };
     if constexpr (HBM) {
-        check_db<LI, 0, CAP>(Ls, t, buf, first, pa, pb, alv, bev, corr, rcap);
-        check_db<LI, 1, CAP>(Ls, t + CfgDB::THREADS, buf, first, pa, pb, alv, bev, corr, rcap);
-        store_r_db<LI>(Rbase, t, buf);
+        check(t, buf, std::integral_constant<int, 0>{});
+        check(t + CF::THREADS, buf, std::integral_constant<int, 1>{});
+        store_r_db<LI, THREE>(Rbase, t, buf);
     } else {
         // unconditional (iteration 0 loads stale words that the next layer
         // ignores, and a branch would only add waits)
-        if constexpr (db_hbm(LN)) load_r_db<LN>(Rbase, t, buf);
-        constexpr int O = db_reg_off(LI);
-        uint32_t w[2 * CfgDB::WMAX];
+        if constexpr (db_hbm(LN)) load_r_db<LN, THREE>(Rbase, t, buf);
+        constexpr int O = CF::reg_off(LI);
+        uint32_t w[2 * CF::WMAX];
 #pragma unroll
         for (int i = 0; i < 2 * W; ++i) w[i] = Rr[O + i];
-        check_db<LI, 0, CAP>(Ls, t, w, first, pa, pb, alv, bev, corr, rcap);
-        check_db<LI, 1, CAP>(Ls, t + CfgDB::THREADS, w, first, pa, pb, alv, bev, corr, rcap);
+        check(t, w, std::integral_constant<int, 0>{});
+        check(t + CF::THREADS, w, std::integral_constant<int, 1>{});
 #pragma unroll
         for (int i = 0; i < 2 * W; ++i) Rr[O + i] = w[i];
     }
     lds_barrier();
 }
 
-template <bool CAP, int... LI>
-__device__ __forceinline__ void iteration_db(float *Ls, uint32_t *Rbase, int t, uint32_t (&buf)[2 * CfgDB::WMAX],
-                                             uint32_t (&Rr)[CfgDB::NREGW], bool first, const float *al,
+template <bool CAP, bool THREE, int... LI>
+__device__ __forceinline__ void iteration_db(float *Ls, uint32_t *Rbase, int t,
+                                             uint32_t (&buf)[2 * CfgDBT<THREE>::WMAX],
+                                             uint32_t (&Rr)[CfgDBT<THREE>::NREGW], bool first, const float *al,
                                              const float *be, const float *pal, const float *pbe, bool corr,
                                              float rcap, std::integer_sequence<int, LI...>) {
-    (layer_db<LI, CAP>(Ls, Rbase, t, buf, Rr, first, al, be, pal, pbe, corr, rcap), ...);
+    (layer_db<LI, CAP, THREE>(Ls, Rbase, t, buf, Rr, first, al, be, pal, pbe, corr, rcap), ...);
 }
 
 template <int LI>
@@ -1272,23 +1356,25 @@ __device__ __forceinline__ bool fails_from_db(const float *Ls, int t) {
     }
 }
 
-// CAP: r_max > 0; CORR: the min* correction of g (both fixed per launch)
-template <bool CAP, bool CORR>
+// CAP: r_max > 0; CORR: the two-minimum min* correction of g; THREE: the
+// three-minimum correction (all fixed per launch)
+template <bool CAP, bool CORR, bool THREE>
 __global__ void __launch_bounds__(CfgDB::THREADS) k_d_big(MultiParams P) {
+    using CF = CfgDBT<THREE>;
     __shared__ __attribute__((aligned(16))) float Ls[CfgDB::N];  // all 160 KiB of LDS
     const Run run = load_run(P);
     const int t = threadIdx.x, lane = t & 63;
-    uint32_t *work = P.work + (size_t)blockIdx.x * CfgDB::WORK_DWORDS;
+    uint32_t *work = P.work + (size_t)blockIdx.x * DB_WORK_STRIDE;
     uint32_t *Rbase = work;
-    uint32_t *ctl = work + CfgDB::CTL;  // [0], [1]: syndrome flags by iteration parity; [2]: queue broadcast
+    uint32_t *ctl = work + CF::CTL;  // [0], [1]: syndrome flags by iteration parity; [2]: queue broadcast
     constexpr bool corr = CORR;
     const float rcap = P.d_rmax;
     unsigned long long blocks = 0, fe = 0, its = 0;
-    uint32_t buf[2 * CfgDB::WMAX], Rr[CfgDB::NREGW];
+    uint32_t buf[2 * CF::WMAX], Rr[CF::NREGW];
 #pragma unroll
-    for (int i = 0; i < 2 * CfgDB::WMAX; ++i) buf[i] = 0u;
+    for (int i = 0; i < 2 * CF::WMAX; ++i) buf[i] = 0u;
 #pragma unroll
-    for (int i = 0; i < CfgDB::NREGW; ++i) Rr[i] = 0u;
+    for (int i = 0; i < CF::NREGW; ++i) Rr[i] = 0u;
     for (;;) {
         if (t == 0) ctl_store(ctl + 2, min(atomicAdd(P.queue, 1u), run.n_cw));
         __syncthreads();
@@ -1308,7 +1394,8 @@ __global__ void __launch_bounds__(CfgDB::THREADS) k_d_big(MultiParams P) {
         while (it < P.max_iter) {
             // the previous iteration's row rebuilds the stored messages
             const int row = d_row(P, it), prow = d_row(P, it > 0 ? it - 1 : 0);
-            iteration_db<CAP>(Ls, Rbase, t, buf, Rr, it == 0, P.d_alpha + row * CfgDB::NR, P.d_beta + row * CfgDB::NR,
+            iteration_db<CAP, THREE>(Ls, Rbase, t, buf, Rr, it == 0, P.d_alpha + row * CfgDB::NR,
+                                     P.d_beta + row * CfgDB::NR,
                          P.d_alpha + prow * CfgDB::NR, P.d_beta + prow * CfgDB::NR, corr, rcap,
                          std::make_integer_sequence<int, NLAYERS>{});
             ++it;
@@ -1345,20 +1432,21 @@ __global__ void __launch_bounds__(CfgDB::THREADS) k_d_big(MultiParams P) {
 }
 
 hipError_t launch_db(const MultiParams &p, int grid, hipStream_t st) {
-    const bool cap = p.d_rmax > 0.0f, corr = p.d_corr != 0;
-    auto k = cap ? (corr ? &k_d_big<true, true> : &k_d_big<true, false>)
-                 : (corr ? &k_d_big<false, true> : &k_d_big<false, false>);
+    const bool cap = p.d_rmax > 0.0f;
+    auto k = p.d_corr == 2 ? (cap ? &k_d_big<true, true, true> : &k_d_big<false, true, true>)
+             : p.d_corr    ? (cap ? &k_d_big<true, true, false> : &k_d_big<false, true, false>)
+                           : (cap ? &k_d_big<true, false, false> : &k_d_big<false, false, false>);
     hipLaunchKernelGGL(k, dim3(grid), dim3(CfgDB::THREADS), 0, st, p);
     return hipGetLastError();
 }
 
-// the four instances share the launch shape and the LDS that bounds residency
+// the six instances share the launch shape and the LDS that bounds residency
 hipError_t occupancy_db(int *wg_per_cu) {
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(wg_per_cu, k_d_big<true, true>, CfgDB::THREADS, 0);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(wg_per_cu, k_d_big<true, true, false>, CfgDB::THREADS, 0);
 }
 
 hipError_t attributes_db(hipFuncAttributes *a) {
-    return hipFuncGetAttributes(a, reinterpret_cast<const void *>(k_d_big<true, true>));
+    return hipFuncGetAttributes(a, reinterpret_cast<const void *>(k_d_big<true, true, false>));
 }
 #endif
 
@@ -1423,7 +1511,7 @@ const MultiVariant TABLE[] = {
 const MultiVariant TABLE_D[] = {variant_d<CfgD<0>>()};
 #else
 const MultiVariant TABLE_D[] = {MultiVariant{2, 0, CfgDB::C, CfgDB::ORD, CfgDB::THREADS, 1, &launch_db, &occupancy_db,
-                                             &attributes_db, 1, sizeof(uint32_t) * (size_t)CfgDB::WORK_DWORDS}};
+                                             &attributes_db, 1, sizeof(uint32_t) * (size_t)DB_WORK_STRIDE}};
 #endif
 static_assert(C1_OK == !(LDPC_KID == 2 && LDPC_RID == 0), "C = 1 availability");
 

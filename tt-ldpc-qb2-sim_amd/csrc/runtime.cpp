@@ -890,9 +890,6 @@ int ldpc_set_dmin_params(ldpc_ctx *c, const float *alpha, const float *beta, uin
     if (!(r_max >= 0.0f) || !std::isfinite(r_max)) return fail(c, LDPC_ERR_ARG, "r_max must be >= 0");
     if (correction < 0 || correction > 2)
         return fail(c, LDPC_ERR_ARG, "correction: 0 (min-sum), 1 (two minima) or 2 (three minima)");
-    int k_id = -1, rate_id = -1;
-    if (correction == 2 && ldpc::ar4ja_ids(c->code, k_id, rate_id) && k_id == 2 && rate_id == 0)
-        return fail(c, LDPC_ERR_UNSUPPORTED, "the K=16384 r1/2 Profile D kernel implements correction 0 and 1 only");
     const size_t nb = (size_t)groups * n_params * sizeof(float);
     for (size_t i = 0; i < (size_t)groups * n_params; ++i)
         if (!std::isfinite(alpha[i]) || !std::isfinite(beta[i])) return fail(c, LDPC_ERR_ARG, "non-finite parameter");

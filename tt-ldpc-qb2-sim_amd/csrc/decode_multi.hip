@@ -1260,10 +1260,15 @@ __device__ __forceinline__ void check_db3(float *Ls, int j, uint32_t *w, bool fi
         const uint32_t sw = s ^ f2u(q[d]);  // bit 31: the message's sign
         const float rr = u2f(f2u(mag) | (sw & 0x80000000u));
         Ls[v[d]] = __fadd_rn(q[d], rr);
-        if constexpr (PACK)
+        if constexpr (PACK) {
             pk |= ((uint32_t)e1 << d) | ((uint32_t)e2 << (6 + d)) | ((sw >> 31) << (12 + d));
-        else
+            // fold the two equality masks into pk here: left to the scheduler,
+            // the twelve 64-bit masks of a check stay live to the end and
+            // spill to VGPR lanes
+            asm volatile("" : "+v"(pk));
+        } else {
             w[H + 2 * d] = f2u(rr);
+        }
     }
     if constexpr (PACK) {
         w[H] = f2u(o23);

@@ -6,7 +6,8 @@ the iteration sum) carries the whole tally; far above it nearly every
 codeword converges at the first or second iteration, so the latch and the
 refill run once per codeword.  Counters and histograms bit for bit, for the
 specialised r1/2 K=1024 kernel, a wave-mode and a group-mode nine-code kernel,
-Profile R (and NMS) and the learned decoder's kernels (Profile D).
+Profile R (and NMS) and the learned decoder's kernels (Profile D, both
+corrections).
 """
 import os
 
@@ -70,9 +71,11 @@ def test_profile_r_far_from_the_waterfall(oracle, profile, ebn0, max_iter, n):
 
 @pytest.mark.parametrize("k,ebn0,max_iter,n", [(1024, -3.0, 8, 256), (1024, 12.0, 30, 512),
                                                (16384, -3.0, 4, 8), (16384, 12.0, 20, 16)])
-def test_profile_d_far_from_the_waterfall(oracle, k, ebn0, max_iter, n):
-    """The learned decoder (Profile D, correction on) against its oracle; K=16384
-    runs its own kernel (messages partly in an HBM workspace)."""
+@pytest.mark.parametrize("corr", [1, 2])
+def test_profile_d_far_from_the_waterfall(oracle, k, ebn0, max_iter, n, corr):
+    """The learned decoder (Profile D, two- and three-minimum corrections)
+    against its oracle; K=16384 runs its own kernel (messages partly in an
+    HBM workspace)."""
     import dmin_oracle as DO
     from ldpc_mi355x import codes, decoder
     code = codes.ar4ja(k, "1/2")
@@ -80,13 +83,13 @@ def test_profile_d_far_from_the_waterfall(oracle, k, ebn0, max_iter, n):
     P = DO.n_params(code)
     a = rng.uniform(0.75, 0.95, (1, P)).astype(np.float32)
     b = rng.uniform(0.0, 0.3, (1, P)).astype(np.float32)
-    _case(f"Profile D K={k} {ebn0} dB max_iter {max_iter} n {n}")
+    _case(f"Profile D K={k} {ebn0} dB max_iter {max_iter} n {n} corr {corr}")
     with decoder.Engine(k, "1/2", max_iter=max_iter, profile="D") as eng:
-        eng.set_dmin_params(a, b, True)
+        eng.set_dmin_params(a, b, corr)
         r = eng.run_point(ebn0, n, seed=0xE3, first_cw=3, chunk_cw=n)
     mu, sg = channel_params(ebn0, code.k / code.n_tx)
     llr = oracle.gen_channel(code, 0xE3, 3, n, mu, sg)
-    o = DO.run_point_d(code, llr, a, b, max_iter, corr=True)
+    o = DO.run_point_d(code, llr, a, b, max_iter, corr=corr)
     _same(r, o)
     if ebn0 < 0:
         assert r.frame_errors >= 0.99 * n

@@ -656,7 +656,7 @@ __device__ __forceinline__ float amin_d(float lo, float hi) {
 // past idx2, then min3 = min1 too), equals min*(min2, min3); an edge equal to
 // min2 that is not idx2 means min2 = min3, so min*(min1, min2) =
 // min*(min1, min3).  No index is tracked.
-template <class CF, int LI>
+template <class CF, int LI, bool CAP>
 __device__ __forceinline__ void run_layer_d3(float *Ls, float (&R)[CF::NR], int t, const float *al, const float *be,
                                              float rcap) {
     constexpr ar4ja::Layer Ly = CF::S.layer[LI];
@@ -686,14 +686,14 @@ __device__ __forceinline__ void run_layer_d3(float *Ls, float (&R)[CF::NR], int 
         const float a = fabsf(q[d]);
         const float g = a == m1 ? o23 : (a == m2 ? o13 : o12);
         const float z = __fsub_rn(__fmul_rn(alc[R0 + d], g), bec[R0 + d]);
-        const float mag = fminf(z > 0.0f ? z : 0.0f, rcap);
+        const float mag = CAP ? fminf(z > 0.0f ? z : 0.0f, rcap) : (z > 0.0f ? z : 0.0f);
         const float r = u2f(f2u(mag) | ((s ^ f2u(q[d])) & 0x80000000u));
         Ls[v[d]] = __fadd_rn(q[d], r);
         R[R0 + d] = r;
     }
 }
 
-template <class CF, int LI>
+template <class CF, int LI, bool CAP>
 __device__ __forceinline__ void run_layer_d(float *Ls, float (&R)[CF::NR], int t, const float *al, const float *be,
                                             bool corr, float rcap) {
     constexpr ar4ja::Layer Ly = CF::S.layer[LI];
@@ -734,9 +734,9 @@ __device__ __forceinline__ void run_layer_d(float *Ls, float (&R)[CF::NR], int t
     for (int d = 0; d < D; ++d) {
         const float g = d == i1 ? m2 : other;
         const float z = __fsub_rn(__fmul_rn(alc[R0 + d], g), bec[R0 + d]);
-        // rcap: r_max, or +inf without a cap; for these non-negative finite
-        // magnitudes min(mag, r_max) is "r_max unless mag < r_max"
-        const float mag = fminf(z > 0.0f ? z : 0.0f, rcap);
+        // CAP: r_max > 0; for these non-negative finite magnitudes
+        // min(mag, r_max) is "r_max unless mag < r_max"
+        const float mag = CAP ? fminf(z > 0.0f ? z : 0.0f, rcap) : (z > 0.0f ? z : 0.0f);
         const float r = u2f(f2u(mag) | ((s ^ f2u(q[d])) & 0x80000000u));
         Ls[v[d]] = __fadd_rn(q[d], r);
         R[R0 + d] = r;
@@ -747,19 +747,19 @@ __device__ __forceinline__ void run_layer_d(float *Ls, float (&R)[CF::NR], int t
 // between the two forms inside one kernel raised the register count of both,
 // 109 -> 162 VGPRs for K=1024 r1/2); otherwise corr selects the two-minimum
 // correction or plain min-sum.
-template <class CF, bool THREE, int... LI>
+template <class CF, bool THREE, bool CAP, int... LI>
 __device__ __forceinline__ void run_iteration_d(float *Ls, float (&R)[CF::NR], int t, const float *al, const float *be,
                                                 bool corr, float rcap, std::integer_sequence<int, LI...>) {
     if constexpr (THREE) {
         if constexpr (CF::WAVE)
-            ((run_layer_d3<CF, LI>(Ls, R, t, al, be, rcap), wave_fence()), ...);
+            ((run_layer_d3<CF, LI, CAP>(Ls, R, t, al, be, rcap), wave_fence()), ...);
         else
-            ((run_layer_d3<CF, LI>(Ls, R, t, al, be, rcap), __syncthreads()), ...);
+            ((run_layer_d3<CF, LI, CAP>(Ls, R, t, al, be, rcap), __syncthreads()), ...);
     } else {
         if constexpr (CF::WAVE)
-            ((run_layer_d<CF, LI>(Ls, R, t, al, be, corr, rcap), wave_fence()), ...);
+            ((run_layer_d<CF, LI, CAP>(Ls, R, t, al, be, corr, rcap), wave_fence()), ...);
         else
-            ((run_layer_d<CF, LI>(Ls, R, t, al, be, corr, rcap), __syncthreads()), ...);
+            ((run_layer_d<CF, LI, CAP>(Ls, R, t, al, be, corr, rcap), __syncthreads()), ...);
     }
 }
 
@@ -837,7 +837,7 @@ __device__ __forceinline__ int d_row(const MultiParams &P, int it) {
     return P.d_groups <= 1 ? 0 : (it < P.d_groups ? it : P.d_groups - 1);
 }
 
-template <class CF, bool THREE>
+template <class CF, bool THREE, bool CAP>
 __global__ void __launch_bounds__(CF::THREADS) k_d_wave(MultiParams P) {
     __shared__ __attribute__((aligned(16))) float smem[CF::CW_PER_WG * CF::N];
     __shared__ uint32_t hist_s[HIST_LDS];
@@ -850,7 +850,7 @@ __global__ void __launch_bounds__(CF::THREADS) k_d_wave(MultiParams P) {
 
     const uint32_t all_slots = (1u << CF::SLOTS) - 1u;
     const bool corr = P.d_corr != 0;
-    const float rcap = P.d_rmax > 0.0f ? P.d_rmax : __builtin_inff();
+    const float rcap = P.d_rmax;
     uint32_t cw = run.n_cw;
     int it = 0;
     float R[CF::NR];
@@ -899,7 +899,7 @@ __global__ void __launch_bounds__(CF::THREADS) k_d_wave(MultiParams P) {
         }
         if (!active) break;
         const int row = d_row(P, it);
-        run_iteration_d<CF, THREE>(Ls, R, t, P.d_alpha + row * CF::NR, P.d_beta + row * CF::NR, corr, rcap,
+        run_iteration_d<CF, THREE, CAP>(Ls, R, t, P.d_alpha + row * CF::NR, P.d_beta + row * CF::NR, corr, rcap,
                             std::make_integer_sequence<int, NLAYERS>{});
         it += 1;
         conv = syndrome_pass_d<CF, 0>(Ls, t, active);
@@ -914,7 +914,7 @@ __global__ void __launch_bounds__(CF::THREADS) k_d_wave(MultiParams P) {
     hist_flush(hist_s, P, CF::THREADS);
 }
 
-template <class CF, bool THREE>
+template <class CF, bool THREE, bool CAP>
 __global__ void __launch_bounds__(CF::THREADS) k_d_group(MultiParams P) {
     __shared__ __attribute__((aligned(16))) float Ls[CF::N];
     __shared__ uint32_t hist_s[HIST_LDS];
@@ -926,7 +926,7 @@ __global__ void __launch_bounds__(CF::THREADS) k_d_group(MultiParams P) {
     const int t = threadIdx.x;
     for (int i = t; i < HIST_LDS; i += CF::THREADS) hist_s[i] = 0u;
     const bool corr = P.d_corr != 0;
-    const float rcap = P.d_rmax > 0.0f ? P.d_rmax : __builtin_inff();
+    const float rcap = P.d_rmax;
     unsigned long long blocks = 0, fe = 0, be = 0, its = 0;
     float R[CF::NR];
     __syncthreads();
@@ -950,7 +950,7 @@ __global__ void __launch_bounds__(CF::THREADS) k_d_group(MultiParams P) {
         bool ok = false;
         while (it < P.max_iter) {
             const int row = d_row(P, it);
-            run_iteration_d<CF, THREE>(Ls, R, t, P.d_alpha + row * CF::NR, P.d_beta + row * CF::NR, corr, rcap,
+            run_iteration_d<CF, THREE, CAP>(Ls, R, t, P.d_alpha + row * CF::NR, P.d_beta + row * CF::NR, corr, rcap,
                                 std::make_integer_sequence<int, NLAYERS>{});
             ++it;
             scan_fails_from_d<CF, 0>(Ls, t, fail_s + (it & 1));
@@ -1455,24 +1455,25 @@ hipError_t attributes_db(hipFuncAttributes *a) {
 }
 #endif
 
-template <class CF, bool THREE = false>
+template <class CF, bool THREE = false, bool CAP = false>
 constexpr auto kernel_of_d() {
     if constexpr (CF::WAVE)
-        return &k_d_wave<CF, THREE>;
+        return &k_d_wave<CF, THREE, CAP>;
     else
-        return &k_d_group<CF, THREE>;
+        return &k_d_group<CF, THREE, CAP>;
 }
 
-// The grid is sized by the residency of the plain / two-minimum instance;
-// where the three-minimum instance holds fewer workgroups per CU, the ones
-// beyond its residency start as others finish and leave at their first queue
-// draw once the queue is empty (no workgroup waits on another).
+// Instances: three-minimum or not, message cap or not.  The grid is sized
+// by the residency of the uncapped plain / two-minimum instance; where
+// another instance holds fewer workgroups per CU, the ones beyond its
+// residency start as others finish and leave at their first queue draw once
+// the queue is empty (no workgroup waits on another).
 template <class CF>
 hipError_t launch_d(const MultiParams &p, int grid, hipStream_t st) {
-    if (p.d_corr == 2)
-        hipLaunchKernelGGL((kernel_of_d<CF, true>()), dim3(grid), dim3(CF::THREADS), 0, st, p);
-    else
-        hipLaunchKernelGGL(kernel_of_d<CF>(), dim3(grid), dim3(CF::THREADS), 0, st, p);
+    const bool cap = p.d_rmax > 0.0f;
+    auto k = p.d_corr == 2 ? (cap ? kernel_of_d<CF, true, true>() : kernel_of_d<CF, true, false>())
+                           : (cap ? kernel_of_d<CF, false, true>() : kernel_of_d<CF, false, false>());
+    hipLaunchKernelGGL(k, dim3(grid), dim3(CF::THREADS), 0, st, p);
     return hipGetLastError();
 }
 

@@ -80,9 +80,11 @@ def test_backward_matches_torch_autograd(corr, r_max, G):
     assert np.abs(ga).max() > 0 and np.abs(gb).max() > 0
 
 
-def test_gradient_against_float64_finite_differences():
+@pytest.mark.parametrize("corr", [False, 2])
+def test_gradient_against_float64_finite_differences(corr):
     """64 codewords, T = 20, tied parameters, central differences of the
-    float64 oracle with step 1e-3.
+    float64 oracle with step 1e-3; plain min-sum and the three-minimum
+    correction.
 
     The operating point is 2.0 dB with messages capped at 8, where every
     codeword of the batch converges.  A codeword that fails makes the
@@ -94,12 +96,13 @@ def test_gradient_against_float64_finite_differences():
     T = 20
     llr = llr_batch(code, 64, 2.0, 11)
     a, b = rand_params(code, 1, 12)
-    L, _, ga, gb = run_ext(code, llr, a, b, T, False, r_max=8.0)
+    print(f"finite differences K=1024 T=20 corr={corr}", flush=True)
+    L, _, ga, gb = run_ext(code, llr, a, b, T, corr, r_max=8.0)
     assert ((L[:, :code.n_tx] <= 0).sum(1) == 0).all(), "every codeword of the batch converges"
     idx = [(0, p) for p in (0, 5, 11, 17, 26, 33, 41, 52, 59)]
     an, fd = [], []
     for which, g in (("alpha", ga), ("beta", gb)):
-        f = DO.fd_grad(code, llr, a, b, T, which, idx, 1e-3, r_max=8.0)
+        f = DO.fd_grad(code, llr, a, b, T, which, idx, 1e-3, corr=corr, r_max=8.0)
         for k, (gg, p) in enumerate(idx):
             an.append(float(g[gg, p]))
             fd.append(float(f[k]))

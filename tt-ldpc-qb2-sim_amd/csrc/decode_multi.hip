@@ -1264,8 +1264,9 @@ __device__ __forceinline__ void check_db3(float *Ls, int j, uint32_t *w, bool fi
             pk |= ((uint32_t)e1 << d) | ((uint32_t)e2 << (6 + d)) | ((sw >> 31) << (12 + d));
             // fold the two equality masks into pk here: left to the scheduler,
             // the twelve 64-bit masks of a check stay live to the end and
-            // spill to VGPR lanes
-            asm volatile("" : "+v"(pk));
+            // spill to VGPR lanes (not volatile: it need not stay ordered
+            // with the other check's statements)
+            asm("" : "+v"(pk));
         } else {
             w[H + 2 * d] = f2u(rr);
         }

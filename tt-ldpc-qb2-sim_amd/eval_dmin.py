@@ -84,6 +84,8 @@ def main(argv=None):
     ap.add_argument("--target-fe", type=int, default=0, help="stop a point early after this many frame errors")
     ap.add_argument("--iters", type=int, nargs="+", default=[20, 100], help="Profile D iteration caps")
     ap.add_argument("--seed", type=int, default=0x5EED2026)
+    ap.add_argument("--first-cw", type=int, default=0,
+                    help="first codeword id of every Monte Carlo point (continue an earlier run of the same seed)")
     ap.add_argument("--ref-iters", type=int, nargs="+", default=[200], help="Approx-Min* iteration caps")
     ap.add_argument("--forward", action="store_true",
                     help="decode the trained decoder with its training forward (T iterations, no early exit)")
@@ -129,7 +131,7 @@ def main(argv=None):
             while r is None or (r.blocks < args.blocks and not (args.target_fe and r.frame_errors >= args.target_fe)):
                 n = min(args.segment, args.blocks - (r.blocks if r else 0))
                 fe_left = max(args.target_fe - (r.frame_errors if r else 0), 1) if args.target_fe else 0
-                r = eng.run_point(eb, n, target_fe=fe_left, seed=args.seed, first_cw=r.next_cw if r else 0,
+                r = eng.run_point(eb, n, target_fe=fe_left, seed=args.seed, first_cw=r.next_cw if r else args.first_cw,
                                   chunk_cw=1 << 20, result=r)
                 print(f"# {name} {eb} dB: {r.blocks} codewords, {r.frame_errors} FE, {r.kernel_gsps:.1f} Gsps",
                       file=sys.stderr, flush=True)

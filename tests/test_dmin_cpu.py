@@ -214,3 +214,23 @@ def test_soft_fer_is_one_minus_the_product_of_bit_probabilities():
     # [T, B, N]: the mean over iterations as well
     stacked = torch.stack([clean, hopeless])
     assert abs(float(D.soft_fer(stacked, n_tx)) - 0.5) < 1e-9
+
+
+def test_shipped_parameters_load_and_fit_their_codes():
+    """Every shipped parameter file loads without pickling, names its code,
+    holds finite float32 rows of the code's parameter count, and a
+    correction the Profile D kernels implement; the three-minimum files are
+    the ones with correction 2."""
+    import glob
+    import os
+    from ldpc_mi355x import dmin as D
+    files = sorted(glob.glob(os.path.join(os.path.dirname(D.__file__), "data", "dmin_*.npz")))
+    assert len(files) >= 5
+    for f in files:
+        z = np.load(f, allow_pickle=False)
+        code = codes.ar4ja(int(z["k_info"]), str(z["rate"]))
+        P = DO.n_params(code)
+        assert z["alpha"].dtype == np.float32 and z["alpha"].shape[1] == P and z["beta"].shape == z["alpha"].shape
+        assert np.isfinite(z["alpha"]).all() and np.isfinite(z["beta"]).all()
+        assert int(z["correction"]) in (0, 1, 2) and float(z["r_max"]) >= 0.0
+        assert (int(z["correction"]) == 2) == f.endswith("_min3.npz"), f
